@@ -1,0 +1,305 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+``Oracle`` wraps ``liboracle.so`` (the C restatement in rl_oracle.c) and
+``Reference`` wraps ``_ref/libref.so`` (the reference's own RangeLib.h compiled
+from its source tree, see Makefile).  Only the tests, the smoke check and the
+benchmark's CPU-baseline leg use this module.
+
+Arrays follow the reference's conventions: occupancy and distance transform
+are x-major (``a[x*height + y]``, i.e. the transpose of an image ``img[y, x]``),
+poses are C-contiguous float32 (n, 3), tables float64 row-major.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def xmajor(img: np.ndarray) -> np.ndarray:
+    """Image ``img[y, x]`` -> x-major uint8 occupancy ``occ[x, y]``."""
+    return np.ascontiguousarray(np.asarray(img, dtype=np.uint8).T)
+
+
+class _World(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("scale", "angle", "origin_x", "origin_y", "sin_angle", "cos_angle")]
+
+
+class _Method(C.Structure):
+    _fields_ = [("kind", C.c_int), ("width", C.c_int), ("height", C.c_int), ("occ", _u8p),
+                ("edt", _fp), ("max_range", C.c_float), ("world", _World), ("table", _dp),
+                ("table_width", C.c_int)]
+
+
+IDENTITY_WORLD = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+
+
+def _load_oracle():
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle`)")
+    lib = C.CDLL(ORACLE_SO)
+    lib.or_edt.argtypes = [_u8p, C.c_int, C.c_int, _fp]
+    lib.or_rm_calc_range.argtypes = [_fp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _ip]
+    lib.or_rm_calc_range.restype = C.c_float
+    lib.or_bl_calc_range.argtypes = [_u8p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _ip]
+    lib.or_bl_calc_range.restype = C.c_float
+    mp = C.POINTER(_Method)
+    lib.or_calc_range.argtypes = [mp, C.c_float, C.c_float, C.c_float]
+    lib.or_calc_range.restype = C.c_float
+    lib.or_calc_range_many.argtypes = [mp, _fp, _fp, C.c_int, _ip]
+    lib.or_numpy_calc_range.argtypes = [mp, _fp, _fp, C.c_int, _ip]
+    lib.or_numpy_calc_range_angles.argtypes = [mp, _fp, _fp, _fp, C.c_int, C.c_int, _ip]
+    lib.or_eval_sensor_model.argtypes = [mp, _fp, _fp, _dp, C.c_int, C.c_int]
+    lib.or_calc_range_repeat_angles_eval_sensor_model.argtypes = [mp, _fp, _fp, _fp, _dp, C.c_int, C.c_int]
+    lib.or_sinf_many.argtypes = [_fp, _fp, C.c_long]
+    lib.or_cosf_many.argtypes = [_fp, _fp, C.c_long]
+    return lib
+
+
+_ORACLE = None
+
+
+def oracle_lib():
+    global _ORACLE
+    if _ORACLE is None:
+        _ORACLE = _load_oracle()
+    return _ORACLE
+
+
+def edt(img: np.ndarray) -> np.ndarray:
+    """Distance transform of image ``img[y, x]``; returned x-major (W, H) float32."""
+    occ = xmajor(img)
+    w, h = occ.shape
+    out = np.empty((w, h), dtype=np.float32)
+    oracle_lib().or_edt(_ptr(occ, _u8p), w, h, _ptr(out, _fp))
+    return out
+
+
+def libc_sinf(x):
+    x = _f32(x)
+    out = np.empty_like(x)
+    oracle_lib().or_sinf_many(_ptr(x, _fp), _ptr(out, _fp), x.size)
+    return out
+
+
+def libc_cosf(x):
+    x = _f32(x)
+    out = np.empty_like(x)
+    oracle_lib().or_cosf_many(_ptr(x, _fp), _ptr(out, _fp), x.size)
+    return out
+
+
+class Oracle:
+    """One range method (``kind`` 'rm' or 'bl') bound to one map, as
+    ``RayMarching(OMap, max_range)`` / ``BresenhamsLine(OMap, max_range)``."""
+
+    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=IDENTITY_WORLD):
+        self.lib = oracle_lib()
+        self.occ = xmajor(img)
+        self.width, self.height = self.occ.shape
+        self.edt = edt(img) if kind == "rm" else np.zeros((1, 1), dtype=np.float32)
+        self.table = np.zeros((1, 1), dtype=np.float64)
+        self.m = _Method()
+        self.m.kind = 1 if kind == "bl" else 0
+        self.m.width = self.width
+        self.m.height = self.height
+        self.m.occ = _ptr(self.occ, _u8p)
+        self.m.edt = _ptr(self.edt, _fp)
+        self.m.max_range = max_range
+        self.m.world = _World(*[float(v) for v in world])
+        self.m.table = _ptr(self.table, _dp)
+        self.m.table_width = 1
+
+    def set_sensor_model(self, table: np.ndarray):
+        self.table = np.ascontiguousarray(table, dtype=np.float64)
+        assert self.table.shape[0] == self.table.shape[1]
+        self.m.table = _ptr(self.table, _dp)
+        self.m.table_width = self.table.shape[0]
+
+    def calc_range(self, x, y, heading):
+        return self.lib.or_calc_range(C.byref(self.m), x, y, heading)
+
+    def calc_range_many(self, ins, counts=False):
+        """Grid-frame batch (no world conversion)."""
+        ins = _f32(ins)
+        n = ins.shape[0]
+        outs = np.empty(n, dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.int32)
+        self.lib.or_calc_range_many(C.byref(self.m), _ptr(ins, _fp), _ptr(outs, _fp), n, _ptr(cnt, _ip))
+        return (outs, cnt) if counts else outs
+
+    def numpy_calc_range(self, ins, counts=False):
+        ins = _f32(ins)
+        n = ins.shape[0]
+        outs = np.empty(n, dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.int32)
+        self.lib.or_numpy_calc_range(C.byref(self.m), _ptr(ins, _fp), _ptr(outs, _fp), n, _ptr(cnt, _ip))
+        return (outs, cnt) if counts else outs
+
+    def numpy_calc_range_angles(self, ins, angles, counts=False):
+        ins, angles = _f32(ins), _f32(angles)
+        n, m = ins.shape[0], angles.shape[0]
+        outs = np.empty(n * m, dtype=np.float32)
+        cnt = np.zeros(n * m, dtype=np.int32)
+        self.lib.or_numpy_calc_range_angles(C.byref(self.m), _ptr(ins, _fp), _ptr(angles, _fp),
+                                            _ptr(outs, _fp), n, m, _ptr(cnt, _ip))
+        return (outs, cnt) if counts else outs
+
+    def eval_sensor_model(self, obs, ranges, num_rays, num_particles):
+        obs, ranges = _f32(obs), _f32(ranges)
+        outs = np.empty(num_particles, dtype=np.float64)
+        self.lib.or_eval_sensor_model(C.byref(self.m), _ptr(obs, _fp), _ptr(ranges, _fp), _ptr(outs, _dp),
+                                      num_rays, num_particles)
+        return outs
+
+    def calc_range_repeat_angles_eval_sensor_model(self, ins, angles, obs):
+        ins, angles, obs = _f32(ins), _f32(angles), _f32(obs)
+        n, m = ins.shape[0], angles.shape[0]
+        w = np.empty(n, dtype=np.float64)
+        self.lib.or_calc_range_repeat_angles_eval_sensor_model(
+            C.byref(self.m), _ptr(ins, _fp), _ptr(angles, _fp), _ptr(obs, _fp), _ptr(w, _dp), n, m)
+        return w
+
+
+# ---------------------------------------------------------------------------
+# The compiled reference (optional: present only where it was built).
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+_REF = None
+
+
+def ref_lib():
+    global _REF
+    if _REF is None:
+        if not ref_available():
+            raise RuntimeError(f"compiled reference missing: {REF_SO} (run `make -C oracle ref`)")
+        lib = C.CDLL(REF_SO)
+        lib.ref_load_png.argtypes = [C.c_char_p, C.c_float, _ip, _ip, _u8p]
+        lib.ref_load_png.restype = C.c_int
+        lib.ref_edt.argtypes = [_u8p, C.c_int, C.c_int, _fp]
+        lib.ref_create.argtypes = [C.c_int, _u8p, C.c_int, C.c_int, C.c_float, _fp]
+        lib.ref_create.restype = C.c_void_p
+        lib.ref_destroy.argtypes = [C.c_void_p]
+        lib.ref_set_sensor_model.argtypes = [C.c_void_p, _dp, C.c_int]
+        lib.ref_calc_range.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
+        lib.ref_calc_range.restype = C.c_float
+        lib.ref_calc_range_many.argtypes = [C.c_void_p, _fp, _fp, C.c_int]
+        lib.ref_numpy_calc_range.argtypes = [C.c_void_p, _fp, _fp, C.c_int]
+        lib.ref_numpy_calc_range_angles.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int, C.c_int]
+        lib.ref_eval_sensor_model.argtypes = [C.c_void_p, _fp, _fp, _dp, C.c_int, C.c_int]
+        lib.ref_repeat_angles_eval_sensor_model.argtypes = [C.c_void_p, _fp, _fp, _fp, _dp, C.c_int, C.c_int]
+        lib.ref_pf_step_mt.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp, _dp, C.c_int, C.c_int, C.c_int]
+        lib.ref_numpy_calc_range_mt.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int]
+        _REF = lib
+    return _REF
+
+
+def ref_load_png(path: str, threshold: float = 128.0) -> np.ndarray:
+    """OMap(path, threshold) occupancy as an image ``img[y, x]``."""
+    lib = ref_lib()
+    w, h = C.c_int(), C.c_int()
+    if lib.ref_load_png(path.encode(), threshold, C.byref(w), C.byref(h), None) != 0:
+        raise IOError(path)
+    occ = np.empty((w.value, h.value), dtype=np.uint8)
+    lib.ref_load_png(path.encode(), threshold, C.byref(w), C.byref(h), _ptr(occ, _u8p))
+    return np.ascontiguousarray(occ.T)
+
+
+def ref_edt(img: np.ndarray) -> np.ndarray:
+    occ = xmajor(img)
+    w, h = occ.shape
+    out = np.empty((w, h), dtype=np.float32)
+    ref_lib().ref_edt(_ptr(occ, _u8p), w, h, _ptr(out, _fp))
+    return out
+
+
+class Reference:
+    """The reference's RayMarching / BresenhamsLine on one map."""
+
+    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=None):
+        self.lib = ref_lib()
+        occ = xmajor(img)
+        w, h = occ.shape
+        self._world = None if world is None else np.asarray(world, dtype=np.float32)
+        wp = None if self._world is None else _ptr(self._world, _fp)
+        self.h = self.lib.ref_create(1 if kind == "bl" else 0, _ptr(occ, _u8p), w, h, max_range, wp)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def set_sensor_model(self, table):
+        self.table = np.ascontiguousarray(table, dtype=np.float64)
+        self.lib.ref_set_sensor_model(self.h, _ptr(self.table, _dp), self.table.shape[0])
+
+    def calc_range(self, x, y, heading):
+        return self.lib.ref_calc_range(self.h, x, y, heading)
+
+    def calc_range_many(self, ins):
+        ins = _f32(ins)
+        outs = np.empty(ins.shape[0], dtype=np.float32)
+        self.lib.ref_calc_range_many(self.h, _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0])
+        return outs
+
+    def numpy_calc_range(self, ins, threads: int = 1):
+        ins = _f32(ins)
+        outs = np.empty(ins.shape[0], dtype=np.float32)
+        if threads > 1:
+            self.lib.ref_numpy_calc_range_mt(self.h, _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0], threads)
+        else:
+            self.lib.ref_numpy_calc_range(self.h, _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0])
+        return outs
+
+    def numpy_calc_range_angles(self, ins, angles):
+        ins, angles = _f32(ins), _f32(angles)
+        n, m = ins.shape[0], angles.shape[0]
+        outs = np.empty(n * m, dtype=np.float32)
+        self.lib.ref_numpy_calc_range_angles(self.h, _ptr(ins, _fp), _ptr(angles, _fp), _ptr(outs, _fp), n, m)
+        return outs
+
+    def eval_sensor_model(self, obs, ranges, num_rays, num_particles):
+        obs, ranges = _f32(obs), _f32(ranges)
+        outs = np.empty(num_particles, dtype=np.float64)
+        self.lib.ref_eval_sensor_model(self.h, _ptr(obs, _fp), _ptr(ranges, _fp), _ptr(outs, _dp),
+                                       num_rays, num_particles)
+        return outs
+
+    def calc_range_repeat_angles_eval_sensor_model(self, ins, angles, obs):
+        ins, angles, obs = _f32(ins), _f32(angles), _f32(obs)
+        n, m = ins.shape[0], angles.shape[0]
+        w = np.empty(n, dtype=np.float64)
+        self.lib.ref_repeat_angles_eval_sensor_model(self.h, _ptr(ins, _fp), _ptr(angles, _fp), _ptr(obs, _fp),
+                                                     _ptr(w, _dp), n, m)
+        return w
+
+    def pf_step(self, ins, angles, obs, threads: int):
+        """numpy_calc_range_angles + eval_sensor_model split over threads."""
+        ins, angles, obs = _f32(ins), _f32(angles), _f32(obs)
+        n, m = ins.shape[0], angles.shape[0]
+        outs = np.empty(n * m, dtype=np.float32)
+        w = np.empty(n, dtype=np.float64)
+        self.lib.ref_pf_step_mt(self.h, _ptr(ins, _fp), _ptr(angles, _fp), _ptr(obs, _fp), _ptr(outs, _fp),
+                                _ptr(w, _dp), n, m, threads)
+        return outs, w

@@ -1,0 +1,146 @@
+// extern "C" harness around the reference's own CPU implementation
+// (includes/RangeLib.h, compiled from the reference tree where it lies, CPU
+// only: USE_CUDA is left undefined).  TEST INFRASTRUCTURE ONLY: it writes the
+// golden vectors, cross-checks the oracle restatement and serves as the timed
+// CPU baseline.  Output goes to oracle/_ref/ (see Makefile).
+//
+// RangeUtils.h holds non-inline definitions, so RangeLib.h is included in
+// exactly this one translation unit.
+#include "includes/RangeLib.h"
+
+#include <stdint.h>
+#include <string.h>
+
+#include <thread>
+#include <vector>
+
+namespace {
+
+ranges::OMap make_map(const uint8_t* occ, int w, int h, const float* world) {
+  ranges::OMap m(w, h);
+  for (int x = 0; x < w; ++x)
+    for (int y = 0; y < h; ++y) m.grid[x][y] = occ[(size_t)x * h + y] != 0;
+  // PyOMap defaults (RangeLibc.pyx:174-180) unless parameters are given
+  m.world_scale = world ? world[0] : 1.0f;
+  m.world_angle = world ? world[1] : 0.0f;
+  m.world_origin_x = world ? world[2] : 0.0f;
+  m.world_origin_y = world ? world[3] : 0.0f;
+  m.world_sin_angle = world ? world[4] : 0.0f;
+  m.world_cos_angle = world ? world[5] : 1.0f;
+  return m;
+}
+
+struct Handle {
+  int kind;
+  ranges::RangeMethod* method;
+};
+
+}  // namespace
+
+extern "C" {
+
+// OMap(filename, threshold) (RangeLib.h:159-201).  Call with occ == NULL to
+// get the size; returns 0 on success.
+int ref_load_png(const char* path, float threshold, int* w, int* h, uint8_t* occ) {
+  ranges::OMap m(std::string(path), threshold);
+  if (m.error()) return 1;
+  *w = (int)m.width;
+  *h = (int)m.height;
+  if (occ)
+    for (unsigned x = 0; x < m.width; ++x)
+      for (unsigned y = 0; y < m.height; ++y) occ[(size_t)x * m.height + y] = m.grid[x][y] ? 1 : 0;
+  return 0;
+}
+
+// DistanceTransform(OMap*) (RangeLib.h:345-373), x-major output.
+void ref_edt(const uint8_t* occ, int w, int h, float* out) {
+  ranges::OMap m = make_map(occ, w, h, nullptr);
+  ranges::DistanceTransform dt(&m);
+  for (int x = 0; x < w; ++x)
+    for (int y = 0; y < h; ++y) out[(size_t)x * h + y] = dt.grid[x][y];
+}
+
+// kind 0 = RayMarching, 1 = BresenhamsLine; world = 6 floats or NULL.
+void* ref_create(int kind, const uint8_t* occ, int w, int h, float max_range, const float* world) {
+  ranges::OMap m = make_map(occ, w, h, world);
+  Handle* hd = new Handle;
+  hd->kind = kind;
+  if (kind == 1)
+    hd->method = new ranges::BresenhamsLine(m, max_range);
+  else
+    hd->method = new ranges::RayMarching(m, max_range);
+  return hd;
+}
+
+void ref_destroy(void* p) {
+  Handle* hd = (Handle*)p;
+  delete hd->method;
+  delete hd;
+}
+
+void ref_set_sensor_model(void* p, double* table, int k) {
+  ((Handle*)p)->method->set_sensor_model(table, k);
+}
+
+float ref_calc_range(void* p, float x, float y, float heading) {
+  return ((Handle*)p)->method->calc_range(x, y, heading);
+}
+
+// Grid-frame batch (the loop RayMarchingGPU::calc_range_many performs).
+void ref_calc_range_many(void* p, const float* ins, float* outs, int n) {
+  ranges::RangeMethod* m = ((Handle*)p)->method;
+  for (int i = 0; i < n; ++i) outs[i] = m->calc_range(ins[3 * i], ins[3 * i + 1], ins[3 * i + 2]);
+}
+
+void ref_numpy_calc_range(void* p, float* ins, float* outs, int n) {
+  ((Handle*)p)->method->numpy_calc_range(ins, outs, n);
+}
+
+void ref_numpy_calc_range_angles(void* p, float* ins, float* angles, float* outs, int np_, int na) {
+  ((Handle*)p)->method->numpy_calc_range_angles(ins, angles, outs, np_, na);
+}
+
+void ref_eval_sensor_model(void* p, float* obs, float* ranges_, double* outs, int rpp, int np_) {
+  ((Handle*)p)->method->eval_sensor_model(obs, ranges_, outs, rpp, np_);
+}
+
+void ref_repeat_angles_eval_sensor_model(void* p, float* ins, float* angles, float* obs,
+                                         double* weights, int np_, int na) {
+  ((Handle*)p)->method->calc_range_repeat_angles_eval_sensor_model(ins, angles, obs, weights, np_, na);
+}
+
+// Particle-filter step split over `threads` std::threads (the methods are
+// re-entrant with the trace map off): numpy_calc_range_angles followed by
+// eval_sensor_model on each thread's slice of particles.
+void ref_pf_step_mt(void* p, float* ins, float* angles, float* obs, float* outs, double* weights,
+                    int np_, int na, int threads) {
+  ranges::RangeMethod* m = ((Handle*)p)->method;
+  if (threads < 1) threads = 1;
+  std::vector<std::thread> pool;
+  int per = (np_ + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int b = t * per, e = std::min(np_, b + per);
+    if (b >= e) break;
+    pool.emplace_back([=]() {
+      m->numpy_calc_range_angles(ins + 3 * (size_t)b, angles, outs + (size_t)b * na, e - b, na);
+      m->eval_sensor_model(obs, outs + (size_t)b * na, weights + b, na, e - b);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// numpy_calc_range split over threads.
+void ref_numpy_calc_range_mt(void* p, float* ins, float* outs, int n, int threads) {
+  ranges::RangeMethod* m = ((Handle*)p)->method;
+  if (threads < 1) threads = 1;
+  std::vector<std::thread> pool;
+  int per = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int b = t * per, e = std::min(n, b + per);
+    if (b >= e) break;
+    pool.emplace_back([=]() { m->numpy_calc_range(ins + 3 * (size_t)b, outs + b, e - b); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+}  // extern "C"

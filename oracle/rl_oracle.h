@@ -1,0 +1,91 @@
+/* CPU oracle for the batched ray-casting path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's CPU algorithms, op for op, so that
+ * the HIP library's results can be checked against it.  Only the test suite,
+ * the smoke check and the CPU-baseline leg of the benchmark load it; the
+ * product path never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the range_libc source tree).  Build with -ffp-contract=off and without
+ * -ffast-math: every float operation must round exactly as the reference's.
+ *
+ * Pinning (which independent check fixes each function):
+ *   or_edt              golden vectors from the compiled reference (maps/*.png
+ *                       and seeded maps), brute-force exact EDT on small maps,
+ *                       all-free / all-occupied closed forms.
+ *   or_rm_calc_range    golden vectors from the compiled reference; invariants
+ *                       (0 <= r <= R, hit cell has EDT 0, r = |cell - origin|).
+ *   or_bl_calc_range    golden vectors from the compiled reference; axis-aligned
+ *                       rays reduce to a row/column scan (brute force).
+ *   or_numpy_*          golden vectors (identity and non-identity world params);
+ *                       identity-parameter closed form of the transform.
+ *   or_eval_sensor_*    golden vectors; constant-table closed form c^M.
+ */
+#ifndef RL_ORACLE_H
+#define RL_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* World-frame parameters of an OMap (RangeLib.h:134-141). */
+typedef struct {
+  float scale, angle, origin_x, origin_y, sin_angle, cos_angle;
+} or_world;
+
+/* One range method bound to a map (RangeMethod state, RangeLib.h:682-688). */
+typedef struct {
+  int kind;            /* 0 = RayMarching, 1 = BresenhamsLine */
+  int width, height;   /* grid[x][y], x < width, y < height */
+  const uint8_t* occ;  /* occupancy, x-major: occ[x*height + y] */
+  const float* edt;    /* distance transform, x-major (RayMarching only) */
+  float max_range;
+  or_world world;
+  const double* table; /* sensor model, row-major K x K (row = observed) */
+  int table_width;
+} or_method;
+
+/* DistanceTransform(OMap*) (RangeLib.h:345-373). */
+void or_edt(const uint8_t* occ, int width, int height, float* out);
+
+/* RayMarching::calc_range (RangeLib.h:927-962); *reads = EDT reads. */
+float or_rm_calc_range(const float* edt, int width, int height, float max_range, float x, float y,
+                       float heading, int* reads);
+
+/* BresenhamsLine::calc_range (RangeLib.h:696-769); *visits = cells tested. */
+float or_bl_calc_range(const uint8_t* occ, int width, int height, float max_range, float x,
+                       float y, float heading, int* visits);
+
+float or_calc_range(const or_method* m, float x, float y, float heading);
+
+/* RayMarchingGPU::calc_range_many semantics (grid frame, no transform;
+ * RangeLib.h:819-831 -> kernels.cu:19-57 with the CPU step constant). */
+void or_calc_range_many(const or_method* m, const float* ins, float* outs, int n, int* reads);
+
+/* RangeMethod::numpy_calc_range (RangeLib.h:439-480). */
+void or_numpy_calc_range(const or_method* m, const float* ins, float* outs, int n, int* reads);
+
+/* RangeMethod::numpy_calc_range_angles (RangeLib.h:482-520). */
+void or_numpy_calc_range_angles(const or_method* m, const float* ins, const float* angles,
+                                float* outs, int num_particles, int num_angles, int* reads);
+
+/* RangeMethod::eval_sensor_model (RangeLib.h:533-555). */
+void or_eval_sensor_model(const or_method* m, const float* obs, const float* ranges, double* outs,
+                          int rays_per_particle, int particles);
+
+/* RangeMethod::calc_range_repeat_angles_eval_sensor_model (RangeLib.h:558-612). */
+void or_calc_range_repeat_angles_eval_sensor_model(const or_method* m, const float* ins,
+                                                   const float* angles, const float* obs,
+                                                   double* weights, int num_particles,
+                                                   int num_angles);
+
+/* The C library's sinf/cosf applied elementwise (for trig parity tests). */
+void or_sinf_many(const float* x, float* out, long n);
+void or_cosf_many(const float* x, float* out, long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
