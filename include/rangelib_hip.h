@@ -1,0 +1,152 @@
+/* rangelib_hip.h -- C ABI of the MI355X (gfx950) ray-casting library.
+ *
+ * Drop-in replacement for the reference's host/device seam (class
+ * RayMarchingCUDA, includes/CudaRangeLib.h:11-61 and includes/kernels.cu:199-315)
+ * and for the CPU-only batch methods of RangeMethod / BresenhamsLine
+ * (includes/RangeLib.h:412-769) that the Cython module binds
+ * (pywrapper/RangeLibc.pyx:25-92).  Plain pointers and sizes only; every
+ * function returns RL_OK (0) or an error code and never throws; the message of
+ * the last failure on the calling thread is rl_last_error().
+ *
+ * Conventions (identical to the reference's):
+ *   occupancy / distance transform  x-major: a[x*height + y], x < width
+ *   poses                           float32 triples (x, y, theta), C order
+ *   sensor table                    float64 K x K row-major, row = observed range
+ *   ranges                          float32; weights float64
+ * Entry points without a _dev suffix take host pointers and return when the
+ * results are in host memory (the reference's synchronous semantics).  The
+ * _dev twins take device pointers, are ordered on the context's stream and
+ * do not synchronise.
+ *
+ * Thread safety: calls on one context are serialised by a context mutex;
+ * distinct contexts are independent.
+ */
+#ifndef RANGELIB_HIP_H
+#define RANGELIB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RL_ABI_VERSION 1
+
+#define RL_OK 0
+#define RL_ERR_ARG 1     /* bad argument: null pointer, size, unknown method */
+#define RL_ERR_HIP 2     /* HIP runtime error (message has the HIP error) */
+#define RL_ERR_STATE 3   /* call not valid in the object's state */
+#define RL_ERR_IO 4      /* missing or corrupt data file */
+#define RL_ERR_NOMEM 5   /* device or host allocation failed */
+
+/* Range methods: RayMarching / RayMarchingGPU (RangeLib.h:774-969) and
+ * BresenhamsLine (RangeLib.h:691-772). */
+#define RL_METHOD_RM 0
+#define RL_METHOD_BL 1
+
+/* Coordinate frames of a batch. */
+#define RL_FRAME_GRID 0    /* no conversion: RayMarchingGPU::calc_range_many (RangeLib.h:819-831) */
+#define RL_FRAME_WORLD 1   /* RangeMethod::numpy_calc_range (RangeLib.h:439-480) */
+#define RL_FRAME_ANGLES 2  /* RangeMethod::numpy_calc_range_angles (RangeLib.h:482-520) */
+
+/* rl_map_create_ex flags */
+#define RL_MAP_EDT_ENVELOPE_ONLY 1u  /* run the lower-envelope pass on columns too (test hook) */
+
+typedef struct rl_ctx rl_ctx;
+typedef struct rl_map rl_map;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rl_last_error(void);
+int rl_abi_version(void);
+
+/* --- context ------------------------------------------------------------ */
+/* Binds a HIP device, creates a stream, uploads the sinf/cosf exception table.
+ * Replaces the device set-up of RayMarchingCUDA::RayMarchingCUDA (kernels.cu:205-218). */
+int rl_create(int device, rl_ctx** out);
+/* Replaces RayMarchingCUDA::~RayMarchingCUDA (kernels.cu:220-222). */
+int rl_destroy(rl_ctx* ctx);
+/* Orders all later work of the context on `hip_stream` (a hipStream_t; NULL
+ * restores the context's own stream).  No counterpart in the reference. */
+int rl_set_stream(rl_ctx* ctx, void* hip_stream);
+/* Waits for the context's stream (cudaDeviceSynchronize, kernels.cu:241). */
+int rl_sync(rl_ctx* ctx);
+
+/* --- maps ----------------------------------------------------------------- */
+/* Uploads an occupancy grid and builds its distance transform on the GPU.
+ * Replaces DistanceTransform(OMap*) (RangeLib.h:345-373) plus the H2D copy of
+ * RayMarchingCUDA::RayMarchingCUDA (kernels.cu:210-216).  World parameters
+ * default to the identity (RangeLibc.pyx:174-180). */
+int rl_map_create(rl_ctx* ctx, const uint8_t* occ_xmajor, int width, int height, rl_map** out);
+int rl_map_create_ex(rl_ctx* ctx, const uint8_t* occ_xmajor, int width, int height,
+                     unsigned flags, rl_map** out);
+int rl_map_destroy(rl_map* map);
+/* OMap world_* fields (RangeLib.h:134-141) as RayMarchingCUDA::set_conversion_params
+ * consumes them (CudaRangeLib.h:24-35). */
+int rl_map_set_world(rl_map* map, float scale, float angle, float origin_x, float origin_y,
+                     float sin_angle, float cos_angle);
+/* RangeMethod::set_sensor_model / RayMarchingCUDA::set_sensor_table
+ * (RangeLib.h:523-532, 878-888; kernels.cu:224-229). */
+int rl_map_set_sensor_table(rl_map* map, const double* table, int table_width);
+/* Copies the distance transform to host memory, x-major (DistanceTransform::grid). */
+int rl_map_get_edt(rl_map* map, float* out_xmajor);
+/* Device pointers of the map's distance transform and occupancy (x-major). */
+int rl_map_device_buffers(rl_map* map, const float** d_edt, const uint8_t** d_occ);
+/* Milliseconds the last distance-transform build took on the device. */
+int rl_map_edt_build_ms(rl_map* map, float* ms);
+
+/* --- single query (host) ---------------------------------------------------- */
+/* <Method>::calc_range (RangeLib.h:696, 927); grid frame. */
+int rl_calc_range(rl_map* map, int method, float max_range, float x, float y, float heading,
+                  float* out);
+
+/* --- batches, host pointers, synchronous ----------------------------------- */
+/* RayMarchingGPU::calc_range_many (RangeLib.h:819-831 -> kernels.cu:232-242). */
+int rl_calc_range_many(rl_map* map, int method, float max_range, const float* ins, float* outs,
+                       int n);
+/* numpy_calc_range (RangeLib.h:439-480, 837-853 -> kernels.cu:245-259). */
+int rl_numpy_calc_range(rl_map* map, int method, float max_range, const float* ins, float* outs,
+                        int n);
+/* numpy_calc_range_angles (RangeLib.h:482-520, 855-874 -> kernels.cu:263-279). */
+int rl_numpy_calc_range_angles(rl_map* map, int method, float max_range, const float* ins,
+                               const float* angles, float* outs, int num_particles,
+                               int num_angles);
+/* RangeMethod::eval_sensor_model (RangeLib.h:533-555); needs a sensor table. */
+int rl_eval_sensor_model(rl_map* map, const float* obs, const float* ranges, double* outs,
+                         int rays_per_particle, int particles);
+/* calc_range_repeat_angles_eval_sensor_model (CPU RangeLib.h:558-612; the GPU
+ * path RangeLib.h:892-910 -> kernels.cu:281-315 is a stub in the reference). */
+int rl_calc_range_repeat_angles_eval_sensor_model(rl_map* map, int method, float max_range,
+                                                  const float* ins, const float* angles,
+                                                  const float* obs, double* weights,
+                                                  int num_particles, int num_angles);
+
+/* --- batches, device pointers, stream-ordered -------------------------------- */
+int rl_calc_range_many_dev(rl_map* map, int method, float max_range, const float* d_ins,
+                           float* d_outs, int n);
+int rl_numpy_calc_range_dev(rl_map* map, int method, float max_range, const float* d_ins,
+                            float* d_outs, int n);
+int rl_numpy_calc_range_angles_dev(rl_map* map, int method, float max_range, const float* d_ins,
+                                   const float* d_angles, float* d_outs, int num_particles,
+                                   int num_angles);
+int rl_eval_sensor_model_dev(rl_map* map, const float* d_obs, const float* d_ranges,
+                             double* d_outs, int rays_per_particle, int particles);
+int rl_calc_range_repeat_angles_eval_sensor_model_dev(rl_map* map, int method, float max_range,
+                                                      const float* d_ins, const float* d_angles,
+                                                      const float* d_obs, double* d_weights,
+                                                      int num_particles, int num_angles);
+
+/* --- instrumentation (device pointers) -------------------------------------- */
+/* Per-ray work counts of a batch: distance-transform reads (RM) or cells
+ * visited (BL).  `frame` is RL_FRAME_*; `angles`/`num_angles` are used by
+ * RL_FRAME_ANGLES only, `num_particles` is the number of poses. */
+int rl_count_steps_dev(rl_map* map, int method, int frame, float max_range, const float* d_ins,
+                       const float* d_angles, int* d_counts, int num_particles, int num_angles);
+/* The library's sinf/cosf (bit-identical to the host C library on |x| <= 64). */
+int rl_debug_sincosf_dev(rl_ctx* ctx, const float* d_x, float* d_sin, float* d_cos, int n);
+/* Device sqrtf as the kernels use it. */
+int rl_debug_sqrtf_dev(rl_ctx* ctx, const float* d_x, float* d_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+"""ctypes binding of librangelib_hip.so (the C ABI in include/rangelib_hip.h).
+
+Loading the shared library needs no GPU; creating a context does.  There is
+no CPU fallback: if the library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("RL_LIBRARY", os.path.join(PKG, "librangelib_hip.so"))
+
+RL_METHOD_RM = 0
+RL_METHOD_BL = 1
+RL_FRAME_GRID = 0
+RL_FRAME_WORLD = 1
+RL_FRAME_ANGLES = 2
+RL_MAP_EDT_ENVELOPE_ONLY = 1
+
+_vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+# name -> argument types; every function returns int status
+SIGNATURES = {
+    "rl_abi_version": [],
+    "rl_create": [C.c_int, C.POINTER(_vp)],
+    "rl_destroy": [_vp],
+    "rl_set_stream": [_vp, _vp],
+    "rl_sync": [_vp],
+    "rl_map_create": [_vp, _u8p, C.c_int, C.c_int, C.POINTER(_vp)],
+    "rl_map_create_ex": [_vp, _u8p, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)],
+    "rl_map_destroy": [_vp],
+    "rl_map_set_world": [_vp] + [C.c_float] * 6,
+    "rl_map_set_sensor_table": [_vp, _dp, C.c_int],
+    "rl_map_get_edt": [_vp, _fp],
+    "rl_map_device_buffers": [_vp, C.POINTER(_vp), C.POINTER(_vp)],
+    "rl_map_edt_build_ms": [_vp, _fp],
+    "rl_calc_range": [_vp, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _fp],
+    "rl_calc_range_many": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
+    "rl_numpy_calc_range": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
+    "rl_numpy_calc_range_angles": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_eval_sensor_model": [_vp, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_calc_range_repeat_angles_eval_sensor_model": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, _vp,
+                                                      C.c_int, C.c_int],
+    "rl_calc_range_many_dev": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
+    "rl_numpy_calc_range_dev": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
+    "rl_numpy_calc_range_angles_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_eval_sensor_model_dev": [_vp, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_calc_range_repeat_angles_eval_sensor_model_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp,
+                                                          _vp, C.c_int, C.c_int],
+    "rl_count_steps_dev": [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
+    "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
+}
+
+
+class RangeLibError(RuntimeError):
+    """A call into librangelib_hip failed; the message is rl_last_error()."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[rl error {code}] {message}")
+        self.code = code
+
+
+_LIB = None
+_LOCK = threading.Lock()
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _LIB
+    with _LOCK:
+        if _LIB is None:
+            if not os.path.exists(LIB_PATH):
+                raise RangeLibError(-1, f"HIP library not built: {LIB_PATH} "
+                                        "(run `python -m range_libc_amd.build`)")
+            h = C.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = C.c_int
+            h.rl_last_error.argtypes = []
+            h.rl_last_error.restype = C.c_char_p
+            _LIB = h
+    return _LIB
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = lib().rl_last_error()
+        raise RangeLibError(rc, msg.decode() if msg else "unknown error")
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One HIP device and stream (rl_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().rl_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rl_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int | None):
+        """Order work on a raw hipStream_t (e.g. torch.cuda.current_stream().cuda_stream)."""
+        check(lib().rl_set_stream(self.handle, C.c_void_p(stream_handle or 0)))
+
+    def sync(self):
+        check(lib().rl_sync(self.handle))
+
+    def sincosf_dev(self, x_ptr: int, s_ptr: int, c_ptr: int, n: int):
+        check(lib().rl_debug_sincosf_dev(self.handle, _vp(x_ptr), _vp(s_ptr), _vp(c_ptr), n))
+
+    def sqrtf_dev(self, x_ptr: int, out_ptr: int, n: int):
+        check(lib().rl_debug_sqrtf_dev(self.handle, _vp(x_ptr), _vp(out_ptr), n))
+
+
+_DEFAULT: dict[int, Context] = {}
+
+
+def default_context(device: int = 0) -> Context:
+    with _LOCK:
+        ctx = _DEFAULT.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _LOCK:
+            _DEFAULT.setdefault(device, ctx)
+            ctx = _DEFAULT[device]
+    return ctx
+
+
+class DeviceMap:
+    """An occupancy grid resident on the GPU with its distance transform (rl_map).
+
+    ``occ_xmajor`` is uint8 (width, height): ``occ[x, y]`` = grid[x][y].
+    """
+
+    def __init__(self, ctx: Context, occ_xmajor: np.ndarray, flags: int = 0):
+        occ = np.ascontiguousarray(occ_xmajor, dtype=np.uint8)
+        if occ.ndim != 2:
+            raise ValueError("occupancy must be 2-D (width, height)")
+        self.ctx = ctx
+        self.width, self.height = int(occ.shape[0]), int(occ.shape[1])
+        h = C.c_void_p()
+        check(lib().rl_map_create_ex(ctx.handle, occ.ctypes.data_as(_u8p), self.width, self.height,
+                                     flags, C.byref(h)))
+        self.handle = h
+        self.has_table = False
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rl_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------
+    def set_world(self, scale, angle, origin_x, origin_y, sin_angle, cos_angle):
+        check(lib().rl_map_set_world(self.handle, scale, angle, origin_x, origin_y, sin_angle, cos_angle))
+
+    def set_sensor_table(self, table: np.ndarray):
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        check(lib().rl_map_set_sensor_table(self.handle, t.ctypes.data_as(_dp), t.shape[0]))
+        self.has_table = True
+
+    def edt(self) -> np.ndarray:
+        out = np.empty((self.width, self.height), dtype=np.float32)
+        check(lib().rl_map_get_edt(self.handle, out.ctypes.data_as(_fp)))
+        return out
+
+    def edt_build_ms(self) -> float:
+        v = C.c_float()
+        check(lib().rl_map_edt_build_ms(self.handle, C.byref(v)))
+        return v.value
+
+    def device_buffers(self):
+        e, o = C.c_void_p(), C.c_void_p()
+        check(lib().rl_map_device_buffers(self.handle, C.byref(e), C.byref(o)))
+        return e.value, o.value
+
+    # -- host-pointer batches (synchronous) ------------------------------------
+    def calc_range(self, method: int, max_range: float, x: float, y: float, heading: float) -> float:
+        out = C.c_float()
+        check(lib().rl_calc_range(self.handle, method, max_range, x, y, heading, C.byref(out)))
+        return out.value
+
+    def calc_range_many(self, method, max_range, ins, outs, n):
+        check(lib().rl_calc_range_many(self.handle, method, max_range, ptr(ins), ptr(outs), n))
+
+    def numpy_calc_range(self, method, max_range, ins, outs, n):
+        check(lib().rl_numpy_calc_range(self.handle, method, max_range, ptr(ins), ptr(outs), n))
+
+    def numpy_calc_range_angles(self, method, max_range, ins, angles, outs, n, m):
+        check(lib().rl_numpy_calc_range_angles(self.handle, method, max_range, ptr(ins), ptr(angles),
+                                               ptr(outs), n, m))
+
+    def eval_sensor_model(self, obs, ranges, outs, num_rays, num_particles):
+        check(lib().rl_eval_sensor_model(self.handle, ptr(obs), ptr(ranges), ptr(outs), num_rays,
+                                         num_particles))
+
+    def repeat_angles_eval_sensor_model(self, method, max_range, ins, angles, obs, weights, n, m):
+        check(lib().rl_calc_range_repeat_angles_eval_sensor_model(
+            self.handle, method, max_range, ptr(ins), ptr(angles), ptr(obs), ptr(weights), n, m))
+
+    # -- device-pointer batches (stream-ordered, integer addresses) -------------
+    def calc_range_many_dev(self, method, max_range, ins, outs, n):
+        check(lib().rl_calc_range_many_dev(self.handle, method, max_range, _vp(ins), _vp(outs), n))
+
+    def numpy_calc_range_dev(self, method, max_range, ins, outs, n):
+        check(lib().rl_numpy_calc_range_dev(self.handle, method, max_range, _vp(ins), _vp(outs), n))
+
+    def numpy_calc_range_angles_dev(self, method, max_range, ins, angles, outs, n, m):
+        check(lib().rl_numpy_calc_range_angles_dev(self.handle, method, max_range, _vp(ins), _vp(angles),
+                                                   _vp(outs), n, m))
+
+    def eval_sensor_model_dev(self, obs, ranges, outs, num_rays, num_particles):
+        check(lib().rl_eval_sensor_model_dev(self.handle, _vp(obs), _vp(ranges), _vp(outs), num_rays,
+                                             num_particles))
+
+    def repeat_angles_eval_sensor_model_dev(self, method, max_range, ins, angles, obs, weights, n, m):
+        check(lib().rl_calc_range_repeat_angles_eval_sensor_model_dev(
+            self.handle, method, max_range, _vp(ins), _vp(angles), _vp(obs), _vp(weights), n, m))
+
+    def count_steps_dev(self, method, frame, max_range, ins, angles, counts, n, m=0):
+        check(lib().rl_count_steps_dev(self.handle, method, frame, max_range, _vp(ins), _vp(angles or 0),
+                                       _vp(counts), n, m))

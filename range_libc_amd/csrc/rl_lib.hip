@@ -1,0 +1,947 @@
+// librangelib_hip: ray casting on MI355X (gfx950) behind the C ABI of
+// include/rangelib_hip.h.
+//
+// Kernels
+//   k_edt_cols      distance-transform pass 1 (closed form along y, per column)
+//   k_edt_init      f = 0 (occupied) / FLT_MAX (free) for the envelope pass
+//   k_edt_envelope  distance-transform lower envelope, one line per lane,
+//                   replicating vendor/distance_transform.h:1054-1096
+//   k_cast          one ray per lane: ray marching or Bresenham, grid / world /
+//                   repeated-angles frames
+//   k_sensor        eval_sensor_model: one particle per wave segment
+//   k_fused         calc_range_repeat_angles_eval_sensor_model
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off (see build.py).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <libgen.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rangelib_hip.h"
+#include "rl_device.h"
+
+#pragma clang fp contract(off)
+
+using rl::MapDev;
+using rl::TrigDev;
+using rl::WorldDev;
+
+// ---------------------------------------------------------------------------
+// Kernels
+
+// Pass 1 along y for column x = blockIdx.x.  For a 0 / FLT_MAX input the
+// reference's lower envelope yields (y - nearest occupied y)^2, or FLT_MAX for
+// a column without obstacles; every intermediate of the envelope is an exact
+// integer while y^2 < 2^24, i.e. for heights up to 4096, where this closed form
+// is used.
+__global__ __launch_bounds__(256) void k_edt_cols(const uint8_t* __restrict__ occ,
+                                                  float* __restrict__ g, int H) {
+  __shared__ int s_last[256];
+  __shared__ int s_first[256];
+  const int x = blockIdx.x;
+  const int t = threadIdx.x;
+  const uint8_t* col = occ + (size_t)x * H;
+  float* gc = g + (size_t)x * H;
+  const int chunk = (H + 255) / 256;
+  const int y0 = min(H, t * chunk), y1 = min(H, y0 + chunk);
+  int last = -1, first = INT_MAX;
+  for (int y = y0; y < y1; ++y)
+    if (col[y]) {
+      if (first == INT_MAX) first = y;
+      last = y;
+    }
+  s_last[t] = last;
+  s_first[t] = first;
+  __syncthreads();
+  if (t == 0) {  // exclusive prefix max / suffix min over the 256 chunks
+    int run = -1;
+    for (int i = 0; i < 256; ++i) {
+      int v = s_last[i];
+      s_last[i] = run;
+      run = v > run ? v : run;
+    }
+    run = INT_MAX;
+    for (int i = 255; i >= 0; --i) {
+      int v = s_first[i];
+      s_first[i] = run;
+      run = v < run ? v : run;
+    }
+  }
+  __syncthreads();
+  int prev = s_last[t];
+  for (int y = y0; y < y1; ++y) {
+    if (col[y]) prev = y;
+    gc[y] = __int_as_float(prev);  // parked here until the backward sweep
+  }
+  int next = s_first[t];
+  for (int y = y1 - 1; y >= y0; --y) {
+    if (col[y]) next = y;
+    int p = __float_as_int(gc[y]);
+    int dmin = INT_MAX;
+    if (p >= 0) dmin = y - p;
+    if (next != INT_MAX && next - y < dmin) dmin = next - y;
+    gc[y] = (dmin == INT_MAX) ? FLT_MAX : (float)((unsigned)dmin * (unsigned)dmin);
+  }
+}
+
+__global__ void k_edt_init(const uint8_t* __restrict__ occ, float* __restrict__ f, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = occ[i] ? 0.0f : FLT_MAX;
+}
+
+// Lower envelope of parabolas along one line per lane
+// (vendor/distance_transform.h:1054-1096, types as there: float values, double
+// intersections).  Line `l` starts at f + l*ls and steps by `es`; the stacks
+// v[] and z[] of line l live at v[k*n_lines + l] so that neighbouring lanes
+// touch neighbouring words.
+__global__ __launch_bounds__(64) void k_edt_envelope(const float* __restrict__ f,
+                                                     float* __restrict__ D, int n_lines, int n,
+                                                     long long es, long long ls,
+                                                     int* __restrict__ vbuf,
+                                                     double* __restrict__ zbuf, int do_sqrt) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_lines) return;
+  const float* fl = f + (long long)l * ls;
+  float* Dl = D + (long long)l * ls;
+  int* v = vbuf + l;
+  double* z = zbuf + l;
+  const long long L = n_lines;
+  if (n == 1) {
+    float val = fl[0];
+    Dl[0] = do_sqrt ? __builtin_sqrtf(val) : val;
+    return;
+  }
+  unsigned k = 0;
+  double s = 0.0;
+  v[0] = 0;
+  z[0] = -DBL_MAX;
+  z[L] = DBL_MAX;
+  for (unsigned q = 1; q < (unsigned)n; ++q) {
+    float fq = fl[(long long)q * es] + (float)(q * q);
+    ++k;
+    do {
+      --k;
+      unsigned vk = (unsigned)v[(long long)k * L];
+      float rhs = fl[(long long)vk * es] + (float)(vk * vk);
+      s = ((double)fq - (double)rhs) / ((double)(2u * q) - (double)(2u * vk));
+    } while (s <= z[(long long)k * L]);
+    ++k;
+    v[(long long)k * L] = (int)q;
+    z[(long long)k * L] = s;
+    z[(long long)(k + 1) * L] = DBL_MAX;
+  }
+  k = 0;
+  for (unsigned q = 0; q < (unsigned)n; ++q) {
+    while (z[(long long)(k + 1) * L] < (double)q) ++k;
+    unsigned vk = (unsigned)v[(long long)k * L];
+    float dq = (float)q - (float)vk;
+    float val = fl[(long long)vk * es] + dq * dq;
+    Dl[(long long)q * es] = do_sqrt ? __builtin_sqrtf(val) : val;
+  }
+}
+
+// Pose of ray r in the frame the method consumes.  FRAME 0 = grid (no
+// conversion), 1 = world (RangeLib.h:463-475), 2 = repeated angles
+// (RangeLib.h:506-518).  Returns the factor the range is multiplied by.
+template <int FRAME>
+__device__ __forceinline__ float ray_pose(const MapDev& m, const float* __restrict__ ins,
+                                          const float* __restrict__ angles, int r, int num_angles,
+                                          float* x, float* y, float* heading) {
+  if (FRAME == 0) {
+    *x = ins[3 * (size_t)r];
+    *y = ins[3 * (size_t)r + 1];
+    *heading = ins[3 * (size_t)r + 2];
+    return 1.0f;
+  }
+  int p = r, a = 0;
+  if (FRAME == 2) {
+    p = r / num_angles;
+    a = r - p * num_angles;
+  }
+  float gx, gy;
+  rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
+  float theta = -ins[3 * (size_t)p + 2] + m.world.rotation_const;
+  if (FRAME == 2) theta = theta - angles[a];
+  *x = gy;  // calc_range(y, x, theta): the grid axes swap (RangeLib.h:475)
+  *y = gx;
+  *heading = theta;
+  return m.world.scale;
+}
+
+template <int METHOD, bool COUNT>
+__device__ __forceinline__ float cast_one(const MapDev& m, const TrigDev& trig, float max_range,
+                                          float x, float y, float heading, int* cnt) {
+  float s, c;
+  rl::sincos_libc(heading, trig, &s, &c);
+  if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
+  return rl::rm_march<COUNT>(m.edt, m.width, m.height, max_range, x, y, c, s, cnt);
+}
+
+template <int METHOD, int FRAME, bool COUNT>
+__global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_range,
+                                              const float* __restrict__ ins,
+                                              const float* __restrict__ angles,
+                                              float* __restrict__ outs, int* __restrict__ counts,
+                                              int n_rays, int num_angles) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rays) return;
+  float x, y, h;
+  float scale = ray_pose<FRAME>(m, ins, angles, r, num_angles, &x, &y, &h);
+  int cnt = 0;
+  float d = cast_one<METHOD, COUNT>(m, trig, max_range, x, y, h, &cnt);
+  if (COUNT)
+    counts[r] = cnt;
+  else
+    outs[r] = (FRAME == 0) ? d : d * scale;
+}
+
+// Particles are laid out in wave segments of S lanes (S a power of two,
+// <= 64); lane j of a segment handles beams j, j + S, ...  The weight is the
+// product over beams in beam order, as the reference's loop forms it
+// (RangeLib.h:542-554), so it is bit-identical: the terms are broadcast
+// across the segment with shuffles and multiplied sequentially.
+__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt) {
+  for (int jj = 0; jj < cnt; ++jj) w *= __shfl(term, base + jj, 64);
+  return w;
+}
+
+__global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
+                                                const float* __restrict__ ranges,
+                                                double* __restrict__ outs,
+                                                const double* __restrict__ table, int K,
+                                                float inv_scale, int M, int N, int S) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int per_wave = 64 / S;
+  const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
+  const int p = wave * per_wave + seg;
+  const bool valid = p < N;
+  const float hi = (float)((double)(float)K - 1.0);
+  double w = 1.0;
+  for (int c0 = 0; c0 < M; c0 += S) {
+    int j = c0 + j0;
+    double term = 1.0;
+    if (valid && j < M) {
+      float r = rl::clamp_table(obs[j] * inv_scale, hi);
+      float d = rl::clamp_table(ranges[(size_t)p * M + j] * inv_scale, hi);
+      term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
+    }
+    w = segment_product(w, term, base, min(S, M - c0));
+  }
+  if (valid && j0 == 0) outs[p] = w;
+}
+
+template <int METHOD>
+__global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max_range,
+                                               const float* __restrict__ ins,
+                                               const float* __restrict__ angles,
+                                               const float* __restrict__ obs,
+                                               const double* __restrict__ table, int K,
+                                               double* __restrict__ weights, int N, int M, int S) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int per_wave = 64 / S;
+  const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
+  const int p = wave * per_wave + seg;
+  const bool valid = p < N;
+  const float hi = (float)((double)(float)K - 1.0);
+  float gx = 0.0f, gy = 0.0f, theta = 0.0f;
+  if (valid) {  // RangeLib.h:589-597
+    rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
+    theta = -ins[3 * (size_t)p + 2] + m.world.rotation_const;
+  }
+  double w = 1.0;
+  for (int c0 = 0; c0 < M; c0 += S) {
+    int j = c0 + j0;
+    double term = 1.0;
+    if (valid && j < M) {
+      int unused;
+      // the range stays in grid units here: no world_scale (RangeLib.h:602-603)
+      float d = cast_one<METHOD, false>(m, trig, max_range, gy, gx, theta - angles[j], &unused);
+      d = rl::clamp_table(d, hi);
+      float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
+      term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
+    }
+    w = segment_product(w, term, base, min(S, M - c0));
+  }
+  if (valid && j0 == 0) weights[p] = w;
+}
+
+__global__ void k_debug_sincos(TrigDev trig, const float* __restrict__ x, float* __restrict__ s,
+                               float* __restrict__ c, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a, b;
+  rl::sincos_libc(x[i], trig, &a, &b);
+  s[i] = a;
+  c[i] = b;
+}
+
+__global__ void k_debug_sqrt(const float* __restrict__ x, float* __restrict__ out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = __builtin_sqrtf(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) return fail(RL_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHECK_ARG(cond, msg)                       \
+  do {                                             \
+    if (!(cond)) return fail(RL_ERR_ARG, "%s", msg); \
+  } while (0)
+
+struct TrigFile {
+  bool loaded = false;
+  std::string error;
+  double delta_s = 0, delta_c = 0;
+  std::vector<uint32_t> off_s, off_c;
+  std::vector<uint16_t> keys_s, keys_c;
+  std::string libc_version;
+};
+
+struct TrigHeader {
+  char magic[8];
+  uint32_t nbuckets, abs_limit, nkeys_sin, nkeys_cos;
+  double delta_sin, delta_cos;
+  char libc_version[32];
+};
+
+std::mutex g_trig_mu;
+TrigFile g_trig;
+
+std::string library_dir() {
+  Dl_info info;
+  if (dladdr((void*)&rl_create, &info) && info.dli_fname) {
+    std::vector<char> p(info.dli_fname, info.dli_fname + strlen(info.dli_fname) + 1);
+    return std::string(dirname(p.data()));
+  }
+  return ".";
+}
+
+// Loads the sinf/cosf exception table once per process.
+const TrigFile* trig_table(std::string* err) {
+  std::lock_guard<std::mutex> lk(g_trig_mu);
+  if (g_trig.loaded) return &g_trig;
+  const char* env = getenv("RL_TRIG_TABLE");
+  std::string path = env ? std::string(env) : library_dir() + "/_data/trig_exc.bin";
+  FILE* fp = fopen(path.c_str(), "rb");
+  if (!fp) {
+    *err = "trig exception table not found: " + path + " (run the build)";
+    return nullptr;
+  }
+  TrigHeader h;
+  bool ok = fread(&h, sizeof(h), 1, fp) == 1 && memcmp(h.magic, "RLTRIG01", 8) == 0 &&
+            h.nbuckets == RLT_NBUCKETS && h.abs_limit == RLT_ABS_LIMIT;
+  if (ok) {
+    g_trig.off_s.resize(h.nbuckets + 1);
+    g_trig.off_c.resize(h.nbuckets + 1);
+    g_trig.keys_s.resize(h.nkeys_sin);
+    g_trig.keys_c.resize(h.nkeys_cos);
+    ok = fread(g_trig.off_s.data(), 4, h.nbuckets + 1, fp) == h.nbuckets + 1 &&
+         fread(g_trig.keys_s.data(), 2, h.nkeys_sin, fp) == h.nkeys_sin &&
+         fread(g_trig.off_c.data(), 4, h.nbuckets + 1, fp) == h.nbuckets + 1 &&
+         fread(g_trig.keys_c.data(), 2, h.nkeys_cos, fp) == h.nkeys_cos;
+    ok = ok && g_trig.off_s.back() == h.nkeys_sin && g_trig.off_c.back() == h.nkeys_cos;
+  }
+  fclose(fp);
+  if (!ok) {
+    *err = "corrupt trig exception table: " + path;
+    g_trig = TrigFile();
+    return nullptr;
+  }
+  g_trig.delta_s = h.delta_sin;
+  g_trig.delta_c = h.delta_cos;
+  h.libc_version[31] = 0;
+  g_trig.libc_version = h.libc_version;
+  g_trig.loaded = true;
+  return &g_trig;
+}
+
+int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+
+int segment_size(int m) {
+  int s = 1;
+  while (s < m && s < 64) s <<= 1;
+  return s;
+}
+
+}  // namespace
+
+struct rl_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  void* trig_buf = nullptr;
+  TrigDev trig{};
+  std::mutex mu;
+  void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
+
+  // Device staging buffer `slot` of at least `bytes` (host-pointer entry points).
+  int staging(int slot, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 16;
+    if (stage_cap[slot] < bytes) {
+      if (stage[slot]) HIP_TRY(hipFree(stage[slot]));
+      stage[slot] = nullptr;
+      stage_cap[slot] = 0;
+      size_t cap = bytes + bytes / 4;
+      if (hipMalloc(&stage[slot], cap) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RL_ERR_NOMEM, "hipMalloc of %zu staging bytes failed", cap);
+      }
+      stage_cap[slot] = cap;
+    }
+    *out = stage[slot];
+    return RL_OK;
+  }
+};
+
+struct rl_map {
+  rl_ctx* ctx = nullptr;
+  int width = 0, height = 0;
+  float* d_edt = nullptr;
+  uint8_t* d_occ = nullptr;
+  double* d_table = nullptr;
+  int table_width = 0;
+  WorldDev world{};
+  float edt_ms = 0.0f;
+
+  MapDev dev() const { return MapDev{d_edt, d_occ, width, height, world}; }
+};
+
+namespace {
+
+void set_world(rl_map* m, float scale, float angle, float ox, float oy, float sin_a, float cos_a) {
+  // RangeLib.h:442-450 / CudaRangeLib.h:26-34
+  m->world.inv_scale = (float)(1.0 / (double)scale);
+  m->world.scale = scale;
+  m->world.origin_x = ox;
+  m->world.origin_y = oy;
+  m->world.sin_a = sin_a;
+  m->world.cos_a = cos_a;
+  m->world.rotation_const = (float)(-1.0 * (double)angle - 3.0 * 3.14159265358979323846 / 2.0);
+}
+
+int check_method(int method) {
+  if (method != RL_METHOD_RM && method != RL_METHOD_BL)
+    return fail(RL_ERR_ARG, "unknown range method %d", method);
+  return RL_OK;
+}
+
+#define TRY(expr)              \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != RL_OK) return rc_; \
+  } while (0)
+
+template <int METHOD, int FRAME, bool COUNT>
+int launch_cast_t(rl_map* m, float max_range, const float* ins, const float* angles, float* outs,
+                  int* counts, int n_rays, int num_angles) {
+  if (n_rays == 0) return RL_OK;
+  hipLaunchKernelGGL((k_cast<METHOD, FRAME, COUNT>), dim3(grid_for(n_rays, 256)), dim3(256), 0,
+                     m->ctx->stream, m->dev(), m->ctx->trig, max_range, ins, angles, outs, counts,
+                     n_rays, num_angles);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+template <int FRAME, bool COUNT>
+int launch_cast_f(rl_map* m, int method, float max_range, const float* ins, const float* angles,
+                  float* outs, int* counts, int n_rays, int num_angles) {
+  if (method == RL_METHOD_BL)
+    return launch_cast_t<RL_METHOD_BL, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  return launch_cast_t<RL_METHOD_RM, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
+}
+
+int launch_cast(rl_map* m, int method, int frame, bool count, float max_range, const float* ins,
+                const float* angles, float* outs, int* counts, int n_rays, int num_angles) {
+  TRY(check_method(method));
+  if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  if (frame == RL_FRAME_GRID)
+    return count ? launch_cast_f<0, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
+                 : launch_cast_f<0, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  if (frame == RL_FRAME_WORLD)
+    return count ? launch_cast_f<1, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
+                 : launch_cast_f<1, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  if (frame == RL_FRAME_ANGLES)
+    return count ? launch_cast_f<2, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
+                 : launch_cast_f<2, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  return fail(RL_ERR_ARG, "unknown frame %d", frame);
+}
+
+int launch_sensor(rl_map* m, const float* obs, const float* ranges, double* outs, int M, int N) {
+  if (!m->d_table) return fail(RL_ERR_STATE, "no sensor model set (call set_sensor_model first)");
+  if (N == 0) return RL_OK;
+  int S = segment_size(M);
+  long long waves = ((long long)N + 64 / S - 1) / (64 / S);
+  hipLaunchKernelGGL(k_sensor, dim3(grid_for(waves * 64, 256)), dim3(256), 0, m->ctx->stream, obs,
+                     ranges, outs, m->d_table, m->table_width, m->world.inv_scale, M, N, S);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+int launch_fused(rl_map* m, int method, float max_range, const float* ins, const float* angles,
+                 const float* obs, double* weights, int N, int M) {
+  TRY(check_method(method));
+  if (!m->d_table) return fail(RL_ERR_STATE, "no sensor model set (call set_sensor_model first)");
+  if (N == 0) return RL_OK;
+  int S = segment_size(M);
+  long long waves = ((long long)N + 64 / S - 1) / (64 / S);
+  dim3 grid(grid_for(waves * 64, 256)), block(256);
+  if (method == RL_METHOD_BL)
+    hipLaunchKernelGGL(k_fused<RL_METHOD_BL>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
+                       max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
+  else
+    hipLaunchKernelGGL(k_fused<RL_METHOD_RM>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
+                       max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+// Builds the distance transform of m->d_occ into m->d_edt.
+int build_edt(rl_map* m, unsigned flags) {
+  rl_ctx* c = m->ctx;
+  const int W = m->width, H = m->height;
+  const size_t N = (size_t)W * H;
+  float* g = nullptr;
+  int* v = nullptr;
+  double* z = nullptr;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = RL_OK;
+  do {
+    if (hipMalloc(&g, N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&v, N * sizeof(int)) != hipSuccess ||
+        hipMalloc(&z, (N + (size_t)W + (size_t)H) * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(RL_ERR_NOMEM, "device allocation for the %dx%d distance transform failed", W, H);
+      break;
+    }
+    if (hipEventRecord(e0, c->stream) != hipSuccess) {
+      rc = fail(RL_ERR_HIP, "hipEventRecord failed");
+      break;
+    }
+    if (H <= 4096 && !(flags & RL_MAP_EDT_ENVELOPE_ONLY)) {
+      hipLaunchKernelGGL(k_edt_cols, dim3(W), dim3(256), 0, c->stream, m->d_occ, g, H);
+    } else {
+      hipLaunchKernelGGL(k_edt_init, dim3(grid_for((long long)N, 256)), dim3(256), 0, c->stream,
+                         m->d_occ, m->d_edt, N);
+      hipLaunchKernelGGL(k_edt_envelope, dim3(grid_for(W, 64)), dim3(64), 0, c->stream,
+                         (const float*)m->d_edt, g, W, H, 1LL, (long long)H, v, z, 0);
+    }
+    hipLaunchKernelGGL(k_edt_envelope, dim3(grid_for(H, 64)), dim3(64), 0, c->stream,
+                       (const float*)g, m->d_edt, H, W, (long long)H, 1LL, v, z, 1);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      rc = fail(RL_ERR_HIP, "distance transform: %s", hipGetErrorString(e));
+      break;
+    }
+    (void)hipEventElapsedTime(&m->edt_ms, e0, e1);
+  } while (0);
+  if (g) (void)hipFree(g);
+  if (v) (void)hipFree(v);
+  if (z) (void)hipFree(z);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int use_device(rl_ctx* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return RL_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* rl_last_error(void) { return g_err.c_str(); }
+
+int rl_abi_version(void) { return RL_ABI_VERSION; }
+
+int rl_create(int device, rl_ctx** out) {
+  CHECK_ARG(out, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(RL_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+  std::string err;
+  const TrigFile* tf = trig_table(&err);
+  if (!tf) return fail(RL_ERR_IO, "%s", err.c_str());
+  HIP_TRY(hipSetDevice(device));
+  rl_ctx* c = new rl_ctx;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(RL_ERR_HIP, "hipStreamCreate failed");
+  }
+  c->stream = c->own_stream;
+  // One device buffer: off_s | off_c | keys_s | keys_c
+  size_t b_off = tf->off_s.size() * 4, b_ks = tf->keys_s.size() * 2, b_kc = tf->keys_c.size() * 2;
+  size_t total = 2 * b_off + b_ks + b_kc + 64;
+  if (hipMalloc(&c->trig_buf, total) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return fail(RL_ERR_NOMEM, "hipMalloc of the trig table failed");
+  }
+  char* base = (char*)c->trig_buf;
+  hipError_t e = hipMemcpy(base, tf->off_s.data(), b_off, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(base + b_off, tf->off_c.data(), b_off, hipMemcpyHostToDevice);
+  if (e == hipSuccess && b_ks) e = hipMemcpy(base + 2 * b_off, tf->keys_s.data(), b_ks, hipMemcpyHostToDevice);
+  if (e == hipSuccess && b_kc) e = hipMemcpy(base + 2 * b_off + b_ks, tf->keys_c.data(), b_kc, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(c->trig_buf);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return fail(RL_ERR_HIP, "trig table upload: %s", hipGetErrorString(e));
+  }
+  c->trig.off_s = (const uint32_t*)base;
+  c->trig.off_c = (const uint32_t*)(base + b_off);
+  c->trig.key_s = (const uint16_t*)(base + 2 * b_off);
+  c->trig.key_c = (const uint16_t*)(base + 2 * b_off + b_ks);
+  c->trig.delta_s = tf->delta_s;
+  c->trig.delta_c = tf->delta_c;
+  *out = c;
+  return RL_OK;
+}
+
+int rl_destroy(rl_ctx* c) {
+  if (!c) return RL_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 6; ++i)
+    if (c->stage[i]) (void)hipFree(c->stage[i]);
+  if (c->trig_buf) (void)hipFree(c->trig_buf);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return RL_OK;
+}
+
+int rl_set_stream(rl_ctx* c, void* stream) {
+  CHECK_ARG(c, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return RL_OK;
+}
+
+int rl_sync(rl_ctx* c) {
+  CHECK_ARG(c, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
+
+int rl_map_create_ex(rl_ctx* c, const uint8_t* occ, int width, int height, unsigned flags,
+                     rl_map** out) {
+  CHECK_ARG(c && occ && out, "null argument");
+  *out = nullptr;
+  if (width < 1 || height < 1 || width > 65535 || height > 65535)
+    return fail(RL_ERR_ARG, "map size %dx%d outside 1..65535", width, height);
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  rl_map* m = new rl_map;
+  m->ctx = c;
+  m->width = width;
+  m->height = height;
+  set_world(m, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f);  // RangeLibc.pyx:174-180
+  const size_t N = (size_t)width * height;
+  if (hipMalloc(&m->d_occ, N) != hipSuccess || hipMalloc(&m->d_edt, N * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    rl_map_destroy(m);
+    return fail(RL_ERR_NOMEM, "device allocation for a %dx%d map failed", width, height);
+  }
+  if (hipMemcpyAsync(m->d_occ, occ, N, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    rl_map_destroy(m);
+    return fail(RL_ERR_HIP, "occupancy upload failed");
+  }
+  int rc = build_edt(m, flags);
+  if (rc != RL_OK) {
+    std::string msg = g_err;
+    rl_map_destroy(m);
+    return fail(rc, "%s", msg.c_str());
+  }
+  *out = m;
+  return RL_OK;
+}
+
+int rl_map_create(rl_ctx* c, const uint8_t* occ, int width, int height, rl_map** out) {
+  return rl_map_create_ex(c, occ, width, height, 0u, out);
+}
+
+int rl_map_destroy(rl_map* m) {
+  if (!m) return RL_OK;
+  if (m->ctx) {
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+  }
+  if (m->d_occ) (void)hipFree(m->d_occ);
+  if (m->d_edt) (void)hipFree(m->d_edt);
+  if (m->d_table) (void)hipFree(m->d_table);
+  delete m;
+  return RL_OK;
+}
+
+int rl_map_set_world(rl_map* m, float scale, float angle, float ox, float oy, float sin_a,
+                     float cos_a) {
+  CHECK_ARG(m, "map is NULL");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  set_world(m, scale, angle, ox, oy, sin_a, cos_a);
+  return RL_OK;
+}
+
+int rl_map_set_sensor_table(rl_map* m, const double* table, int k) {
+  CHECK_ARG(m && table, "null argument");
+  CHECK_ARG(k >= 1, "table width must be >= 1");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  if (m->d_table) {
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    HIP_TRY(hipFree(m->d_table));
+    m->d_table = nullptr;
+  }
+  size_t bytes = (size_t)k * k * sizeof(double);
+  if (hipMalloc(&m->d_table, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    m->d_table = nullptr;
+    return fail(RL_ERR_NOMEM, "hipMalloc of a %dx%d sensor table failed", k, k);
+  }
+  HIP_TRY(hipMemcpyAsync(m->d_table, table, bytes, hipMemcpyHostToDevice, m->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+  m->table_width = k;
+  return RL_OK;
+}
+
+int rl_map_get_edt(rl_map* m, float* out) {
+  CHECK_ARG(m && out, "null argument");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  HIP_TRY(hipMemcpyAsync(out, m->d_edt, (size_t)m->width * m->height * sizeof(float),
+                         hipMemcpyDeviceToHost, m->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+  return RL_OK;
+}
+
+int rl_map_device_buffers(rl_map* m, const float** d_edt, const uint8_t** d_occ) {
+  CHECK_ARG(m, "map is NULL");
+  if (d_edt) *d_edt = m->d_edt;
+  if (d_occ) *d_occ = m->d_occ;
+  return RL_OK;
+}
+
+int rl_map_edt_build_ms(rl_map* m, float* ms) {
+  CHECK_ARG(m && ms, "null argument");
+  *ms = m->edt_ms;
+  return RL_OK;
+}
+
+// --- host-pointer batches ----------------------------------------------------
+
+static int host_cast(rl_map* m, int method, int frame, float max_range, const float* ins,
+                     const float* angles, float* outs, int num_poses, int num_angles) {
+  CHECK_ARG(m && ins && outs, "null argument");
+  CHECK_ARG(num_poses >= 0, "negative batch size");
+  if (frame == RL_FRAME_ANGLES) {
+    CHECK_ARG(angles, "angles is NULL");
+    CHECK_ARG(num_angles >= 1, "num_angles must be >= 1");
+  }
+  TRY(check_method(method));
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  long long n_rays = frame == RL_FRAME_ANGLES ? (long long)num_poses * num_angles : num_poses;
+  if (n_rays > 0x7fffffffLL) return fail(RL_ERR_ARG, "batch of %lld rays exceeds 2^31-1", n_rays);
+  if (n_rays == 0) return RL_OK;
+  void *d_ins, *d_ang = nullptr, *d_out;
+  TRY(c->staging(0, (size_t)num_poses * 3 * sizeof(float), &d_ins));
+  TRY(c->staging(1, (size_t)n_rays * sizeof(float), &d_out));
+  HIP_TRY(hipMemcpyAsync(d_ins, ins, (size_t)num_poses * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  if (frame == RL_FRAME_ANGLES) {
+    TRY(c->staging(2, (size_t)num_angles * sizeof(float), &d_ang));
+    HIP_TRY(hipMemcpyAsync(d_ang, angles, (size_t)num_angles * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  }
+  TRY(launch_cast(m, method, frame, false, max_range, (const float*)d_ins, (const float*)d_ang,
+                  (float*)d_out, nullptr, (int)n_rays, num_angles));
+  HIP_TRY(hipMemcpyAsync(outs, d_out, (size_t)n_rays * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
+
+int rl_calc_range(rl_map* m, int method, float max_range, float x, float y, float heading,
+                  float* out) {
+  CHECK_ARG(out, "out is NULL");
+  float in[3] = {x, y, heading};
+  return host_cast(m, method, RL_FRAME_GRID, max_range, in, nullptr, out, 1, 0);
+}
+
+int rl_calc_range_many(rl_map* m, int method, float max_range, const float* ins, float* outs, int n) {
+  return host_cast(m, method, RL_FRAME_GRID, max_range, ins, nullptr, outs, n, 0);
+}
+
+int rl_numpy_calc_range(rl_map* m, int method, float max_range, const float* ins, float* outs, int n) {
+  return host_cast(m, method, RL_FRAME_WORLD, max_range, ins, nullptr, outs, n, 0);
+}
+
+int rl_numpy_calc_range_angles(rl_map* m, int method, float max_range, const float* ins,
+                               const float* angles, float* outs, int np_, int na) {
+  return host_cast(m, method, RL_FRAME_ANGLES, max_range, ins, angles, outs, np_, na);
+}
+
+int rl_eval_sensor_model(rl_map* m, const float* obs, const float* ranges, double* outs, int M,
+                         int N) {
+  CHECK_ARG(m && obs && ranges && outs, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad rays_per_particle / particles");
+  CHECK_ARG((long long)M * N <= 0x7fffffffLL, "batch too large");
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (N == 0) return RL_OK;
+  void *d_obs, *d_rng, *d_w;
+  TRY(c->staging(2, (size_t)M * sizeof(float), &d_obs));
+  TRY(c->staging(3, (size_t)M * N * sizeof(float), &d_rng));
+  TRY(c->staging(4, (size_t)N * sizeof(double), &d_w));
+  HIP_TRY(hipMemcpyAsync(d_obs, obs, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_rng, ranges, (size_t)M * N * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  TRY(launch_sensor(m, (const float*)d_obs, (const float*)d_rng, (double*)d_w, M, N));
+  HIP_TRY(hipMemcpyAsync(outs, d_w, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
+
+int rl_calc_range_repeat_angles_eval_sensor_model(rl_map* m, int method, float max_range,
+                                                  const float* ins, const float* angles,
+                                                  const float* obs, double* weights, int N, int M) {
+  CHECK_ARG(m && ins && angles && obs && weights, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad num_angles / num_particles");
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (N == 0) return RL_OK;
+  void *d_ins, *d_ang, *d_obs, *d_w;
+  TRY(c->staging(0, (size_t)N * 3 * sizeof(float), &d_ins));
+  TRY(c->staging(2, (size_t)M * sizeof(float), &d_ang));
+  TRY(c->staging(3, (size_t)M * sizeof(float), &d_obs));
+  TRY(c->staging(4, (size_t)N * sizeof(double), &d_w));
+  HIP_TRY(hipMemcpyAsync(d_ins, ins, (size_t)N * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_ang, angles, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_obs, obs, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  TRY(launch_fused(m, method, max_range, (const float*)d_ins, (const float*)d_ang,
+                   (const float*)d_obs, (double*)d_w, N, M));
+  HIP_TRY(hipMemcpyAsync(weights, d_w, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
+
+// --- device-pointer batches ---------------------------------------------------
+
+static int dev_cast(rl_map* m, int method, int frame, float max_range, const float* ins,
+                    const float* angles, float* outs, int* counts, int num_poses, int num_angles) {
+  CHECK_ARG(m && ins && (outs || counts), "null argument");
+  CHECK_ARG(num_poses >= 0, "negative batch size");
+  if (frame == RL_FRAME_ANGLES) {
+    CHECK_ARG(angles, "angles is NULL");
+    CHECK_ARG(num_angles >= 1, "num_angles must be >= 1");
+  }
+  long long n_rays = frame == RL_FRAME_ANGLES ? (long long)num_poses * num_angles : num_poses;
+  if (n_rays > 0x7fffffffLL) return fail(RL_ERR_ARG, "batch of %lld rays exceeds 2^31-1", n_rays);
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  return launch_cast(m, method, frame, counts != nullptr, max_range, ins, angles, outs, counts,
+                     (int)n_rays, num_angles);
+}
+
+int rl_calc_range_many_dev(rl_map* m, int method, float max_range, const float* ins, float* outs, int n) {
+  return dev_cast(m, method, RL_FRAME_GRID, max_range, ins, nullptr, outs, nullptr, n, 0);
+}
+
+int rl_numpy_calc_range_dev(rl_map* m, int method, float max_range, const float* ins, float* outs, int n) {
+  return dev_cast(m, method, RL_FRAME_WORLD, max_range, ins, nullptr, outs, nullptr, n, 0);
+}
+
+int rl_numpy_calc_range_angles_dev(rl_map* m, int method, float max_range, const float* ins,
+                                   const float* angles, float* outs, int np_, int na) {
+  return dev_cast(m, method, RL_FRAME_ANGLES, max_range, ins, angles, outs, nullptr, np_, na);
+}
+
+int rl_eval_sensor_model_dev(rl_map* m, const float* obs, const float* ranges, double* outs, int M,
+                             int N) {
+  CHECK_ARG(m && obs && ranges && outs, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad rays_per_particle / particles");
+  CHECK_ARG((long long)M * N <= 0x7fffffffLL, "batch too large");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  return launch_sensor(m, obs, ranges, outs, M, N);
+}
+
+int rl_calc_range_repeat_angles_eval_sensor_model_dev(rl_map* m, int method, float max_range,
+                                                      const float* ins, const float* angles,
+                                                      const float* obs, double* weights, int N,
+                                                      int M) {
+  CHECK_ARG(m && ins && angles && obs && weights, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad num_angles / num_particles");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  return launch_fused(m, method, max_range, ins, angles, obs, weights, N, M);
+}
+
+int rl_count_steps_dev(rl_map* m, int method, int frame, float max_range, const float* ins,
+                       const float* angles, int* counts, int np_, int na) {
+  CHECK_ARG(counts, "counts is NULL");
+  return dev_cast(m, method, frame, max_range, ins, angles, nullptr, counts, np_, na);
+}
+
+int rl_debug_sincosf_dev(rl_ctx* c, const float* x, float* s, float* co, int n) {
+  CHECK_ARG(c && x && s && co && n >= 0, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (n == 0) return RL_OK;
+  hipLaunchKernelGGL(k_debug_sincos, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->trig, x, s, co, n);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+int rl_debug_sqrtf_dev(rl_ctx* c, const float* x, float* out, int n) {
+  CHECK_ARG(c && x && out && n >= 0, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (n == 0) return RL_OK;
+  hipLaunchKernelGGL(k_debug_sqrt, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, out, n);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""GPU parity: the HIP library against the CPU oracle, bit for bit.
+
+Every test runs the product path (librangelib_hip.so through its C ABI) on
+the device and compares with the oracle restatement of the reference's CPU
+code on the same seeded inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import range_libc_amd as rl
+from oracle import oracle as O
+from range_libc_amd import workloads as W
+from tests.conftest import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU test needs a device"
+    return torch
+
+
+# --------------------------------------------------------------------------- trig / sqrt
+
+def test_sincosf_bit_exact_vs_libc(ctx):
+    torch = _torch()
+    rng = np.random.default_rng(0)
+    two_to_four = np.arange(0x40000000, 0x40800000, dtype=np.uint32).view(np.float32)  # all of [2, 4)
+    xs = np.concatenate([
+        rng.uniform(-64, 64, 3_000_000).astype(np.float32),
+        rng.uniform(-16, 0, 1_000_000).astype(np.float32),
+        two_to_four, -two_to_four,
+        np.array([0.0, -0.0, 1e-30, -1e-30, 64.0, -64.0, math.pi, -math.pi], dtype=np.float32),
+        rng.uniform(-1e4, 1e4, 200_000).astype(np.float32),  # outside the table domain
+    ])
+    x = torch.from_numpy(xs).cuda()
+    s = torch.empty_like(x)
+    c = torch.empty_like(x)
+    ctx.sincosf_dev(x.data_ptr(), s.data_ptr(), c.data_ptr(), x.numel())
+    ctx.sync()
+    s_h, c_h = s.cpu().numpy(), c.cpu().numpy()
+    inside = np.abs(xs) <= 64.0
+    assert_bits_equal(s_h[inside], O.libc_sinf(xs[inside]), "sinf |x|<=64")
+    assert_bits_equal(c_h[inside], O.libc_cosf(xs[inside]), "cosf |x|<=64")
+    # outside the exhaustively tabulated domain: nearest rounding, ~0.1% may differ by 1 ulp
+    out = ~inside
+    ds = np.abs(s_h[out].view(np.int32).astype(np.int64) - O.libc_sinf(xs[out]).view(np.int32))
+    assert ds.max() <= 1 and (ds > 0).mean() < 0.01
+
+
+def test_sqrtf_correctly_rounded(ctx):
+    torch = _torch()
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([np.arange(0, 1 << 24, dtype=np.float32),
+                         rng.uniform(0, 1e7, 2_000_000).astype(np.float32),
+                         np.array([np.finfo(np.float32).max, 0.0, 1e-40], dtype=np.float32)])
+    x = torch.from_numpy(xs).cuda()
+    o = torch.empty_like(x)
+    ctx.sqrtf_dev(x.data_ptr(), o.data_ptr(), x.numel())
+    ctx.sync()
+    assert_bits_equal(o.cpu().numpy(), np.sqrt(xs), "sqrtf")
+
+
+# --------------------------------------------------------------------------- distance transform
+
+def _edt_gpu(ctx, img, flags=0):
+    m = rl.DeviceMap(ctx, O.xmajor(img), flags)
+    return m.edt()
+
+
+@pytest.mark.parametrize("case", ["rect512", "corridor1024", "empty", "full", "single_free",
+                                  "one_px", "tall", "wide", "sparse_cols", "very_tall"])
+def test_edt_bit_exact(ctx, case):
+    rng = np.random.default_rng(7)
+    img = {
+        "rect512": lambda: W.rect_disk_map(512, seed=2),
+        "corridor1024": lambda: W.corridor_map(1024, seed=3),
+        "empty": lambda: np.zeros((37, 53), np.uint8),
+        "full": lambda: np.ones((20, 30), np.uint8),
+        "single_free": lambda: np.pad(np.zeros((1, 1), np.uint8), 3, constant_values=1),
+        "one_px": lambda: np.ones((1, 1), np.uint8),
+        "tall": lambda: (rng.random((700, 13)) < 0.02).astype(np.uint8),
+        "wide": lambda: (rng.random((9, 800)) < 0.01).astype(np.uint8),
+        "sparse_cols": lambda: (rng.random((300, 257)) < 0.0005).astype(np.uint8),
+        "very_tall": lambda: (rng.random((4500, 6)) < 0.003).astype(np.uint8),  # > 4096 rows
+    }[case]()
+    want = O.edt(img)
+    assert_bits_equal(_edt_gpu(ctx, img), want, f"edt {case}")
+    # the full lower-envelope path on columns (used above 4096 rows) agrees too
+    assert_bits_equal(_edt_gpu(ctx, img, rl._lib.RL_MAP_EDT_ENVELOPE_ONLY), want, f"edt envelope {case}")
+
+
+# --------------------------------------------------------------------------- ray casting
+
+@pytest.fixture(scope="module")
+def rect_map():
+    return W.rect_disk_map(512, seed=0)
+
+
+@pytest.fixture(scope="module")
+def corridor_map():
+    return W.corridor_map(2048, seed=1)
+
+
+@pytest.mark.parametrize("kind", ["rm", "bl"])
+def test_calc_range_many_world_frame(kind, rect_map):
+    ins = W.world_poses(rect_map, 100_000, seed=5)
+    cls = rl.PyRayMarchingGPU if kind == "rm" else rl.PyBresenhamsLine
+    meth = cls(rl.PyOMap(rect_map), 500.0)
+    out = np.empty(ins.shape[0], np.float32)
+    meth.calc_range_many(ins, out)
+    assert_bits_equal(out, O.Oracle(rect_map, 500.0, kind).numpy_calc_range(ins), f"{kind} world")
+
+
+@pytest.mark.parametrize("kind", ["rm", "bl"])
+def test_calc_range_many_grid_frame(kind, rect_map):
+    ins = W.grid_poses(rect_map, 50_000, seed=6)
+    cls = rl.PyRayMarchingGPU if kind == "rm" else rl.PyBresenhamsLine
+    meth = cls(rl.PyOMap(rect_map), 500.0)
+    out = np.empty(ins.shape[0], np.float32)
+    meth.calc_range_many_grid(ins, out)
+    assert_bits_equal(out, O.Oracle(rect_map, 500.0, kind).calc_range_many(ins), f"{kind} grid")
+
+
+@pytest.mark.parametrize("kind", ["rm", "bl"])
+def test_repeat_angles(kind, corridor_map):
+    ins = W.world_poses(corridor_map, 4000, seed=7)
+    ang = W.beam_angles(60)
+    cls = rl.PyRayMarchingGPU if kind == "rm" else rl.PyBresenhamsLine
+    meth = cls(rl.PyOMap(corridor_map), 500.0)
+    out = np.empty(4000 * 60, np.float32)
+    meth.calc_range_repeat_angles(ins, ang, out)
+    assert_bits_equal(out, O.Oracle(corridor_map, 500.0, kind).numpy_calc_range_angles(ins, ang),
+                      f"{kind} angles")
+
+
+def test_world_params_non_identity(rect_map):
+    world = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+    omap = rl.PyOMap(rect_map)
+    omap.set_world(*world)
+    # poses whose grid image lies in free space: invert x = R(angle) (x_w - o) / scale
+    g = W.world_poses(rect_map, 20_000, seed=8).astype(np.float64)
+    c, s = math.cos(0.3), math.sin(0.3)
+    xw = (c * g[:, 0] + s * g[:, 1]) * 0.05 - 3.0
+    yw = (-s * g[:, 0] + c * g[:, 1]) * 0.05 + 2.0
+    wins = np.stack([xw, yw, g[:, 2] - 1.0], axis=1).astype(np.float32)
+    for kind, cls in (("rm", rl.PyRayMarchingGPU), ("bl", rl.PyBresenhamsLine)):
+        out = np.empty(wins.shape[0], np.float32)
+        cls(omap, 500.0).calc_range_many(wins, out)
+        assert_bits_equal(out, O.Oracle(rect_map, 500.0, kind, world).numpy_calc_range(wins), f"{kind} world params")
+
+
+def test_single_query_calc_range(rect_map):
+    orc = O.Oracle(rect_map, 300.0, "rm")
+    orb = O.Oracle(rect_map, 300.0, "bl")
+    rm = rl.PyRayMarching(rl.PyOMap(rect_map), 300.0)
+    bl = rl.PyBresenhamsLine(rl.PyOMap(rect_map), 300.0)
+    for x, y, h in W.grid_poses(rect_map, 20, seed=9):
+        assert np.float32(rm.calc_range(x, y, h)) == np.float32(orc.calc_range(x, y, h))
+        assert np.float32(bl.calc_range(x, y, h)) == np.float32(orb.calc_range(x, y, h))
+
+
+# --------------------------------------------------------------------------- sensor model
+
+@pytest.mark.parametrize("m", [1, 7, 60, 64, 65, 120])
+def test_eval_sensor_model_bit_exact(m, corridor_map):
+    rng = np.random.default_rng(m)
+    table = W.sensor_table(501)
+    n = 1000
+    ranges = rng.uniform(-5, 520, n * m).astype(np.float32)
+    obs = rng.uniform(-5, 520, m).astype(np.float32)
+    meth = rl.PyRayMarchingGPU(rl.PyOMap(np.zeros((8, 8), np.uint8)), 500.0)
+    meth.set_sensor_model(table)
+    w = np.empty(n, np.float64)
+    meth.eval_sensor_model(obs, ranges, w, m, n)
+    orc = O.Oracle(np.zeros((8, 8), np.uint8), 500.0)
+    orc.set_sensor_model(table)
+    assert np.array_equal(w, orc.eval_sensor_model(obs, ranges, m, n))
+
+
+@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("bl", 60)])
+def test_fused_sensor_model_bit_exact(kind, m, corridor_map):
+    table = W.sensor_table(501)
+    ins = W.world_poses(corridor_map, 2000, seed=10)
+    ang = W.beam_angles(m)
+    orc = O.Oracle(corridor_map, 500.0, kind)
+    orc.set_sensor_model(table)
+    true_pose = W.world_poses(corridor_map, 1, seed=11)
+    obs = W.noisy_observation(orc.numpy_calc_range_angles(true_pose, ang), seed=12)
+    cls = rl.PyRayMarchingGPU if kind == "rm" else rl.PyBresenhamsLine
+    meth = cls(rl.PyOMap(corridor_map), 500.0)
+    meth.set_sensor_model(table)
+    w = np.empty(2000, np.float64)
+    meth.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs, w)
+    assert np.array_equal(w, orc.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))
+
+
+def test_sensor_model_requires_table():
+    meth = rl.PyRayMarchingGPU(rl.PyOMap(np.zeros((8, 8), np.uint8)), 10.0)
+    with pytest.raises(rl.RangeLibError):
+        meth.eval_sensor_model(np.zeros(4, np.float32), np.zeros(8, np.float32), np.zeros(2), 4, 2)
+
+
+def test_empty_batches(rect_map):
+    meth = rl.PyRayMarchingGPU(rl.PyOMap(rect_map), 100.0)
+    meth.calc_range_many(np.zeros((0, 3), np.float32), np.zeros(0, np.float32))
+    meth.calc_range_repeat_angles(np.zeros((0, 3), np.float32), np.zeros(4, np.float32), np.zeros(0, np.float32))
+
+
+# --------------------------------------------------------------------------- step counts
+
+def test_step_counts_match_oracle(corridor_map):
+    torch = _torch()
+    ins = W.world_poses(corridor_map, 500, seed=13)
+    ang = W.beam_angles(60)
+    m = rl.DeviceMap(rl.default_context(0), O.xmajor(corridor_map))
+    d_ins = torch.from_numpy(ins).cuda()
+    d_ang = torch.from_numpy(ang).cuda()
+    for method, kind in ((rl.RL_METHOD_RM, "rm"), (rl.RL_METHOD_BL, "bl")):
+        cnt = torch.zeros(500 * 60, dtype=torch.int32, device="cuda")
+        m.count_steps_dev(method, rl.RL_FRAME_ANGLES, 500.0, d_ins.data_ptr(), d_ang.data_ptr(),
+                          cnt.data_ptr(), 500, 60)
+        m.ctx.sync()
+        _, want = O.Oracle(corridor_map, 500.0, kind).numpy_calc_range_angles(ins, ang, counts=True)
+        assert np.array_equal(cnt.cpu().numpy(), want)
