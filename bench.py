@@ -1,0 +1,233 @@
+"""Benchmark: one particle-filter step on one MI355X.
+
+Workload (default): 2048x2048 synthetic corridor map, 4000 particles in free
+space, ``calc_range_repeat_angles`` with 60 beams over 270 degrees and
+max_range 500 px (240,000 rays), then ``eval_sensor_model`` with a 501x501
+table against 60 observed ranges.  Inputs are device-resident when the timed
+region starts; a step is the two launches (ray marching over repeated angles,
+sensor model).
+
+Prints one JSON line: rays/s, the ray-marching kernel's fraction of the HBM
+roofline (algorithmic bytes = 4 B per distance-transform read, counted per ray
+on the device, plus pose/angle input and range output), and the reference CPU
+path timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+MAP_SIDE = 2048
+PARTICLES = 4000
+BEAMS = 60
+MAX_RANGE = 500.0
+TABLE = 501
+
+
+def build_workload(seed_map=1, seed_pose=2, seed_true=3, seed_noise=4):
+    from range_libc_amd import workloads as W
+
+    img = W.corridor_map(MAP_SIDE, seed_map)
+    ins = W.world_poses(img, PARTICLES, seed_pose)
+    angles = W.beam_angles(BEAMS)
+    table = W.sensor_table(TABLE)
+    true_pose = W.world_poses(img, 1, seed_true)
+    return img, ins, angles, table, true_pose, seed_noise
+
+
+def cpu_baseline(img, ins, angles, obs, budget_s=10.0):
+    """The reference's CPU path (oracle/_ref, or the oracle port when the
+    compiled reference is absent) on this host's cores: whole particle-filter
+    steps (numpy_calc_range_angles + eval_sensor_model) split over threads,
+    repeated until ~budget_s of wall time."""
+    from oracle import oracle as O
+    from range_libc_amd import workloads as W
+
+    threads = len(os.sched_getaffinity(0))
+    table = W.sensor_table(TABLE)
+    if O.ref_available():
+        kind = "reference"
+        m = O.Reference(img, MAX_RANGE, "rm")
+        m.set_sensor_model(table)
+        step = lambda: m.pf_step(ins, angles, obs, threads)  # noqa: E731
+    else:
+        kind = "port"
+        threads = 1
+        m = O.Oracle(img, MAX_RANGE, "rm")
+        m.set_sensor_model(table)
+
+        def step():
+            r = m.numpy_calc_range_angles(ins, angles)
+            return m.eval_sensor_model(obs, r, BEAMS, PARTICLES)
+    step()  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        step()
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    rays = n * PARTICLES * BEAMS
+    return {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": kind,
+            "sample": f"{n} full steps ({PARTICLES}x{BEAMS} rays + sensor model) in {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import range_libc_amd as rl
+
+    dev = torch.device("cuda", 0)
+    img, ins, angles, table, true_pose, seed_noise = build_workload()
+    from range_libc_amd import workloads as W
+
+    ctx = rl.default_context(0)
+    # a dedicated stream (default priority): torch's current stream is the null
+    # stream, which the library would not order its work on
+    stream = torch.cuda.Stream(device=dev)
+    ctx.set_stream(stream.cuda_stream)
+    torch.cuda.set_stream(stream)
+
+    occ = np.ascontiguousarray(img.T.astype(np.uint8))
+    m = rl.DeviceMap(ctx, occ)
+    edt_ms = m.edt_build_ms()
+    m.set_sensor_table(table)
+
+    d_ins = torch.from_numpy(ins).to(dev)
+    d_ang = torch.from_numpy(angles).to(dev)
+    d_rng = torch.empty(PARTICLES * BEAMS, dtype=torch.float32, device=dev)
+    d_w = torch.empty(PARTICLES, dtype=torch.float64, device=dev)
+    # observation: the GPU's own ranges from a seeded true pose plus noise
+    d_true = torch.from_numpy(true_pose).to(dev)
+    d_tr = torch.empty(BEAMS, dtype=torch.float32, device=dev)
+    m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, d_true.data_ptr(), d_ang.data_ptr(),
+                                  d_tr.data_ptr(), 1, BEAMS)
+    stream.synchronize()
+    obs = W.noisy_observation(d_tr.cpu().numpy(), seed_noise, max_range=MAX_RANGE)
+    d_obs = torch.from_numpy(obs).to(dev)
+
+    # algorithmic bytes of one ray-marching launch: per-ray distance-transform reads
+    d_cnt = torch.empty(PARTICLES * BEAMS, dtype=torch.int32, device=dev)
+    m.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(),
+                      d_cnt.data_ptr(), PARTICLES, BEAMS)
+    stream.synchronize()
+    reads = int(d_cnt.sum().item())
+    alg_bytes = 4 * reads + 4 * PARTICLES * BEAMS + 12 * PARTICLES + 4 * BEAMS
+
+    def cast():
+        m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(),
+                                      d_rng.data_ptr(), PARTICLES, BEAMS)
+
+    def step():
+        cast()
+        m.eval_sensor_model_dev(d_obs.data_ptr(), d_rng.data_ptr(), d_w.data_ptr(), BEAMS, PARTICLES)
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    w_eager = d_w.cpu().numpy().copy()
+
+    # (1) eager: one library call per launch from Python
+    eager_ms = timed(step, args.steps) / args.steps
+
+    # (2) steady state: GROUP steps captured in one HIP graph, replayed
+    GROUP = 10
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        for _ in range(GROUP):
+            step()
+    for _ in range(3):
+        graph.replay()
+    stream.synchronize()
+    assert np.array_equal(d_w.cpu().numpy(), w_eager), "graph replay changed the weights"
+    replays = max(1, args.steps // GROUP)
+    t_start = time.perf_counter()
+    graph_ms = timed(graph.replay, replays) / (replays * GROUP)
+    wall = time.perf_counter() - t_start
+    assert np.array_equal(d_w.cpu().numpy(), w_eager), "graph replay changed the weights"
+
+    # (3) the dominant kernel alone: a spin kernel first keeps the queue full,
+    # so each event pair brackets one launch without host gaps
+    torch.cuda._sleep(100_000_000)
+    pairs = []
+    for _ in range(100):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        cast()
+        b.record(stream)
+        pairs.append((a, b))
+    stream.synchronize()
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in pairs]))
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    step_ms = graph_ms
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_rm_angles_pf_step.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    rays = PARTICLES * BEAMS
+    result = {
+        "metric": "rays/s on 1 GPU (particle-filter step: ray marching over repeated angles + sensor model)",
+        "value": rays / (step_ms * 1e-3),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pf_step_corridor2048_4000x60_r500", "map": f"{MAP_SIDE}x{MAP_SIDE} corridor",
+                   "particles": PARTICLES, "beams": BEAMS, "max_range": MAX_RANGE, "sensor_table": TABLE,
+                   "parallelism": "single_gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_cast<RM, angles>", "kernel_ms": kern_ms,
+                     "alg_bytes_per_launch": alg_bytes, "edt_reads_per_ray": reads / rays},
+        "eager_ms_per_step": eager_ms,
+        "graph_group": GROUP,
+        "edt_build_ms": edt_ms,
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(img, ins, angles, obs, args.cpu_budget)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

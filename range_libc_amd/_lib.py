@@ -80,6 +80,15 @@ def lib():
             if not os.path.exists(LIB_PATH):
                 raise RangeLibError(-1, f"HIP library not built: {LIB_PATH} "
                                         "(run `python -m range_libc_amd.build`)")
+            # One HIP runtime per process: PyTorch ships its own libamdhip64 /
+            # libhsa-runtime64 and links them by unversioned name, so if this
+            # library loaded the system copies first, a later `import torch`
+            # would bring in a second runtime.  Loading torch first makes this
+            # library bind to torch's runtime (same sonames).
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             h = C.CDLL(LIB_PATH)
             for name, args in SIGNATURES.items():
                 fn = getattr(h, name)
