@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
+    ap.add_argument("--box", type=int, default=1, help="stage the map around each particle in LDS (1) or not (0)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 10 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -100,10 +104,21 @@ def main():
     import range_libc_amd as rl
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.rand(n, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 10}))
+        return
     img, ins, angles, table, true_pose, seed_noise = build_workload()
     from range_libc_amd import workloads as W
 
     ctx = rl.default_context(0)
+    ctx.set_option(rl._lib.RL_OPT_XCD_SCHED, args.xcd)
+    ctx.set_option(rl._lib.RL_OPT_LDS_BOX, args.box)
     # a dedicated stream (default priority): torch's current stream is the null
     # stream, which the library would not order its work on
     stream = torch.cuda.Stream(device=dev)
@@ -134,15 +149,21 @@ def main():
                       d_cnt.data_ptr(), PARTICLES, BEAMS)
     stream.synchronize()
     reads = int(d_cnt.sum().item())
-    alg_bytes = 4 * reads + 4 * PARTICLES * BEAMS + 12 * PARTICLES + 4 * BEAMS
+    rays = PARTICLES * BEAMS
+    # 4 B per distance-transform read, pose/angle/observation input, range output,
+    # one 8-B table entry per ray, one 8-B weight per particle
+    alg_bytes = 4 * reads + 12 * PARTICLES + 8 * BEAMS + 4 * rays + 8 * rays + 8 * PARTICLES
 
-    def cast():
+    def cast():  # the whole step: one fused launch
+        m.pf_step_dev(rl.RL_METHOD_RM, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), d_obs.data_ptr(),
+                      d_rng.data_ptr(), d_w.data_ptr(), PARTICLES, BEAMS)
+
+    def two_launch_step():
         m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(),
                                       d_rng.data_ptr(), PARTICLES, BEAMS)
-
-    def step():
-        cast()
         m.eval_sensor_model_dev(d_obs.data_ptr(), d_rng.data_ptr(), d_w.data_ptr(), BEAMS, PARTICLES)
+
+    step = cast
 
     def timed(fn, reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -154,9 +175,16 @@ def main():
         return e0.elapsed_time(e1)
 
     for _ in range(args.warmup):
-        step()
+        two_launch_step()
     stream.synchronize()
     w_eager = d_w.cpu().numpy().copy()
+    r_eager = d_rng.cpu().numpy().copy()
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    assert np.array_equal(d_w.cpu().numpy(), w_eager), "fused step differs from the two calls"
+    assert np.array_equal(d_rng.cpu().numpy(), r_eager), "fused step differs from the two calls"
+    two_launch_ms = timed(two_launch_step, args.steps) / args.steps
 
     # (1) eager: one library call per launch from Python
     eager_ms = timed(step, args.steps) / args.steps
@@ -198,7 +226,6 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    rays = PARTICLES * BEAMS
     result = {
         "metric": "rays/s on 1 GPU (particle-filter step: ray marching over repeated angles + sensor model)",
         "value": rays / (step_ms * 1e-3),
@@ -214,12 +241,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "pf_step_corridor2048_4000x60_r500", "map": f"{MAP_SIDE}x{MAP_SIDE} corridor",
                    "particles": PARTICLES, "beams": BEAMS, "max_range": MAX_RANGE, "sensor_table": TABLE,
-                   "parallelism": "single_gpu"},
+                   "parallelism": "single_gpu", "xcd_scheduling": bool(args.xcd), "lds_box": bool(args.box)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_cast<RM, angles>", "kernel_ms": kern_ms,
+                     "kernel": "k_pf_step<RM>", "kernel_ms": kern_ms,
                      "alg_bytes_per_launch": alg_bytes, "edt_reads_per_ray": reads / rays},
         "eager_ms_per_step": eager_ms,
+        "two_launch_eager_ms_per_step": two_launch_ms,
         "graph_group": GROUP,
         "edt_build_ms": edt_ms,
         "wall_s_timed": wall,

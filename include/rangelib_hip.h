@@ -51,6 +51,7 @@ extern "C" {
 
 /* rl_map_create_ex flags */
 #define RL_MAP_EDT_ENVELOPE_ONLY 1u  /* run the lower-envelope pass on columns too (test hook) */
+#define RL_MAP_NO_COMPACT 2u         /* march the fp32 distance map only (no 8-bit index map) */
 
 typedef struct rl_ctx rl_ctx;
 typedef struct rl_map rl_map;
@@ -70,6 +71,13 @@ int rl_destroy(rl_ctx* ctx);
 int rl_set_stream(rl_ctx* ctx, void* hip_stream);
 /* Waits for the context's stream (cudaDeviceSynchronize, kernels.cu:241). */
 int rl_sync(rl_ctx* ctx);
+
+/* Tuning options of a context (no counterpart in the reference; the
+ * reference's only knobs are the compile-time CHUNK_SIZE / NUM_THREADS,
+ * CMakeLists.txt:13-14). */
+#define RL_OPT_XCD_SCHED 1          /* 1: bucket particles by map region per XCD (default 0) */
+#define RL_OPT_LDS_BOX 2            /* 1 (default): stage 64x64 cells around a particle in LDS */
+int rl_set_option(rl_ctx* ctx, int option, long long value);
 
 /* --- maps ----------------------------------------------------------------- */
 /* Uploads an occupancy grid and builds its distance transform on the GPU.
@@ -134,6 +142,14 @@ int rl_calc_range_repeat_angles_eval_sensor_model_dev(rl_map* map, int method, f
                                                       const float* d_ins, const float* d_angles,
                                                       const float* d_obs, double* d_weights,
                                                       int num_particles, int num_angles);
+
+/* numpy_calc_range_angles (RangeLib.h:482-520) then eval_sensor_model
+ * (RangeLib.h:533-555) on its output -- the particle-filter update that
+ * RangeLibc.pyx:328-329 + :334-335 perform as two calls -- in one launch.
+ * Writes both outputs, each bit-identical to the two separate calls. */
+int rl_pf_step_dev(rl_map* map, int method, float max_range, const float* d_ins,
+                   const float* d_angles, const float* d_obs, float* d_ranges, double* d_weights,
+                   int num_particles, int num_angles);
 
 /* --- instrumentation (device pointers) -------------------------------------- */
 /* Per-ray work counts of a batch: distance-transform reads (RM) or cells

@@ -20,6 +20,9 @@ RL_FRAME_GRID = 0
 RL_FRAME_WORLD = 1
 RL_FRAME_ANGLES = 2
 RL_MAP_EDT_ENVELOPE_ONLY = 1
+RL_MAP_NO_COMPACT = 2
+RL_OPT_XCD_SCHED = 1
+RL_OPT_LDS_BOX = 2
 
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)
@@ -33,6 +36,7 @@ SIGNATURES = {
     "rl_destroy": [_vp],
     "rl_set_stream": [_vp, _vp],
     "rl_sync": [_vp],
+    "rl_set_option": [_vp, C.c_int, C.c_longlong],
     "rl_map_create": [_vp, _u8p, C.c_int, C.c_int, C.POINTER(_vp)],
     "rl_map_create_ex": [_vp, _u8p, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)],
     "rl_map_destroy": [_vp],
@@ -54,6 +58,7 @@ SIGNATURES = {
     "rl_eval_sensor_model_dev": [_vp, _vp, _vp, _vp, C.c_int, C.c_int],
     "rl_calc_range_repeat_angles_eval_sensor_model_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp,
                                                           _vp, C.c_int, C.c_int],
+    "rl_pf_step_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int],
     "rl_count_steps_dev": [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
@@ -136,6 +141,9 @@ class Context:
 
     def sync(self):
         check(lib().rl_sync(self.handle))
+
+    def set_option(self, option: int, value: int):
+        check(lib().rl_set_option(self.handle, option, int(value)))
 
     def sincosf_dev(self, x_ptr: int, s_ptr: int, c_ptr: int, n: int):
         check(lib().rl_debug_sincosf_dev(self.handle, _vp(x_ptr), _vp(s_ptr), _vp(c_ptr), n))
@@ -253,6 +261,10 @@ class DeviceMap:
     def repeat_angles_eval_sensor_model_dev(self, method, max_range, ins, angles, obs, weights, n, m):
         check(lib().rl_calc_range_repeat_angles_eval_sensor_model_dev(
             self.handle, method, max_range, _vp(ins), _vp(angles), _vp(obs), _vp(weights), n, m))
+
+    def pf_step_dev(self, method, max_range, ins, angles, obs, ranges, weights, n, m):
+        check(lib().rl_pf_step_dev(self.handle, method, max_range, _vp(ins), _vp(angles), _vp(obs), _vp(ranges),
+                                   _vp(weights), n, m))
 
     def count_steps_dev(self, method, frame, max_range, ins, angles, counts, n, m=0):
         check(lib().rl_count_steps_dev(self.handle, method, frame, max_range, _vp(ins), _vp(angles or 0),

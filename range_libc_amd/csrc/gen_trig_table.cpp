@@ -157,21 +157,34 @@ static int verify(const char* path) {
   }
   Table ts{off_s.data(), keys_s.data(), h.delta_sin};
   Table tc{off_c.data(), keys_c.data(), h.delta_cos};
-  long bad = 0, total = 0;
-#pragma omp parallel for schedule(dynamic, 1 << 16) reduction(+ : bad, total)
+  // the flat bitmaps the device uses, built exactly as the library builds them
+  int es = table_min_exponent(off_s.data(), keys_s.data());
+  int ec = table_min_exponent(off_c.data(), keys_c.data());
+  std::vector<uint8_t> bms(bitmap_bytes(es), 0), bmc(bitmap_bytes(ec), 0);
+  table_to_bitmap(off_s.data(), keys_s.data(), es, bms.data());
+  table_to_bitmap(off_c.data(), keys_c.data(), ec, bmc.data());
+  Bitmap bs{bms.data(), es, h.delta_sin};
+  Bitmap bc{bmc.data(), ec, h.delta_cos};
+  long bad = 0, bad_bm = 0, total = 0;
+#pragma omp parallel for schedule(dynamic, 1 << 16) reduction(+ : bad, bad_bm, total)
   for (long u = 0; u <= (long)RLT_ABS_LIMIT; ++u) {
     for (int sg = 0; sg < 2; ++sg) {
       float x = u2f((uint32_t)u | (sg ? 0x80000000u : 0u));
-      float s, c;
+      float s, c, s2, c2;
       sincosf_libc(x, ts, tc, &s, &c);
-      if (f2u(s) != f2u(sinf(x))) ++bad;
-      if (f2u(c) != f2u(cosf(x))) ++bad;
+      sincosf_libc_bm(x, bs, bc, &s2, &c2);
+      float gs = sinf(x), gc = cosf(x);
+      if (f2u(s) != f2u(gs)) ++bad;
+      if (f2u(c) != f2u(gc)) ++bad;
+      if (f2u(s2) != f2u(gs)) ++bad_bm;
+      if (f2u(c2) != f2u(gc)) ++bad_bm;
       ++total;
     }
   }
-  printf("verify: %ld inputs, %ld mismatches (libc %s, table from %s)\n", total, bad,
-         gnu_get_libc_version(), h.libc_version);
-  return bad ? 3 : 0;
+  printf("verify: %ld inputs, %ld mismatches (table), %ld mismatches (bitmap, e_min %d/%d) "
+         "(libc %s, table from %s)\n", total, bad, bad_bm, es, ec, gnu_get_libc_version(),
+         h.libc_version);
+  return (bad || bad_bm) ? 3 : 0;
 }
 
 int main(int argc, char** argv) {

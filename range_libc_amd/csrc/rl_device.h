@@ -16,11 +16,7 @@
 namespace rl {
 
 struct TrigDev {
-  const uint32_t* off_s;
-  const uint16_t* key_s;
-  const uint32_t* off_c;
-  const uint16_t* key_c;
-  double delta_s, delta_c;
+  rltrig::Bitmap sin_exc, cos_exc;  // exception bitmaps of the C library's sinf / cosf
 };
 
 // World-to-grid constants, derived on the host exactly as RangeLib.h:442-450.
@@ -28,17 +24,28 @@ struct WorldDev {
   float inv_scale, scale, origin_x, origin_y, sin_a, cos_a, rotation_const;
 };
 
+// Compact distance map: every exact distance-transform value is sqrtf(n) for
+// an integer n that is a sum of two squares.  The 255 smallest such values
+// (d <= 27.3) are tabulated once (lut); the map stores per cell the index of
+// its value in that table, or RL_IDX8_ESCAPE when the value is not tabulated
+// (the fp32 map is then read).  Cells are stored in 8x8 blocks of 64 bytes,
+// blocks x-major: byte (((x >> 3) * nby + (y >> 3)) << 6) | ((x & 7) << 3) | (y & 7).
+#define RL_IDX8_ESCAPE 255u
+#define RL_LUT_SIZE 256
+
 struct MapDev {
-  const float* edt;    // x-major W*H
-  const uint8_t* occ;  // x-major W*H
+  const float* edt;     // x-major W*H
+  const uint8_t* occ;   // x-major W*H
+  const uint8_t* idx8;  // blocked index map (see above)
+  const float* lut;     // RL_LUT_SIZE distances, lut[RL_IDX8_ESCAPE] unused
   int width, height;
+  int nby;              // blocks along y: (height + 7) / 8
+  int use_box;          // stage the 64x64 cells around a particle in LDS
   WorldDev world;
 };
 
 __device__ __forceinline__ void sincos_libc(float h, const TrigDev& t, float* s, float* c) {
-  rltrig::Table ts{t.off_s, t.key_s, t.delta_s};
-  rltrig::Table tc{t.off_c, t.key_c, t.delta_c};
-  rltrig::sincosf_libc(h, ts, tc, s, c);
+  rltrig::sincosf_libc_bm(h, t.sin_exc, t.cos_exc, s, c);
 }
 
 // float -> int as x86 cvttss2si does it: truncation, and INT_MIN for NaN or
@@ -75,6 +82,93 @@ __device__ __forceinline__ float rm_march(const float* __restrict__ edt, int W, 
   }
   if (COUNT) *reads = nr;
   return max_range;
+}
+
+// LDS copy of the 64x64 cells (8x8 compact blocks) around a particle's origin
+// cell, shared by the lanes of the wave that casts its beams.  Box block (i, j)
+// is map block (bx0 + i, by0 + j), stored at lds[(i * 8 + j) * 64] in the
+// compact layout; blocks outside the map are never read (the bounds test comes
+// first).
+#define RL_BOX_BLOCKS 8
+#define RL_BOX_BYTES (RL_BOX_BLOCKS * RL_BOX_BLOCKS * 64)
+struct Box {
+  const uint8_t* lds;  // null: no box
+  int bx0, by0;
+};
+
+// Loads the box around grid cell (x0, y0) into `lds` (RL_BOX_BYTES, one wave:
+// lane l copies box block (l / 8, l % 8)).  Returns the box.
+__device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0, float y0,
+                                         bool active) {
+  Box b{nullptr, 0, 0};
+  if (!active || !(x0 >= 0.0f && x0 < (float)m.width && y0 >= 0.0f && y0 < (float)m.height)) return b;
+  b.bx0 = ((int)x0 >> 3) - RL_BOX_BLOCKS / 2 + 1;
+  b.by0 = ((int)y0 >> 3) - RL_BOX_BLOCKS / 2 + 1;
+  const int lane = threadIdx.x & 63;
+  const int bx = b.bx0 + (lane >> 3), by = b.by0 + (lane & 7);
+  const int nbx = (m.width + 7) >> 3;
+  uint4* dst = reinterpret_cast<uint4*>(lds + lane * 64);
+  if (bx >= 0 && bx < nbx && by >= 0 && by < m.nby) {
+    const uint4* src = reinterpret_cast<const uint4*>(m.idx8 + (((size_t)bx * m.nby + by) << 6));
+    uint4 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+    dst[0] = v0;
+    dst[1] = v1;
+    dst[2] = v2;
+    dst[3] = v3;
+  }
+  // the wave reads what its other lanes wrote: complete the LDS stores first
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  b.lds = lds;
+  return b;
+}
+
+// The same march over the compact map.  `lut` is the distance table staged in
+// LDS.  Valid when |x0|, |y0| and max_range are below 1e9 (checked by the
+// caller): every x0 + dx*t then lies in int range, so the conversion is a
+// plain truncation.  Identical results to rm_march: a cell's code is 0 exactly
+// when its distance is 0, and any other code maps back to the exact distance.
+// Cells inside the particle box are read from LDS, others from the map.
+template <bool COUNT>
+__device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut, const Box& box,
+                                               float max_range, float x0, float y0, float dx,
+                                               float dy, int* reads) {
+  const unsigned W = (unsigned)m.width, H = (unsigned)m.height, nby = (unsigned)m.nby;
+  float t = 0.0f;
+  int nr = 0;
+  while (t < max_range) {
+    int px = (int)(x0 + dx * t);
+    int py = (int)(y0 + dy * t);
+    if ((unsigned)px >= W || (unsigned)py >= H) {
+      if (COUNT) *reads = nr;
+      return max_range;
+    }
+    unsigned ux = (unsigned)px, uy = (unsigned)py;
+    unsigned inner = ((ux & 7u) << 3) | (uy & 7u);
+    unsigned ib = (unsigned)((int)(ux >> 3) - box.bx0), jb = (unsigned)((int)(uy >> 3) - box.by0);
+    unsigned code;
+    if (box.lds && ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS)
+      code = box.lds[((ib * RL_BOX_BLOCKS + jb) << 6) | inner];
+    else
+      code = m.idx8[((size_t)((ux >> 3) * nby + (uy >> 3)) << 6) | inner];
+    if (COUNT) ++nr;
+    if (code == 0u) {
+      float xd = (float)px - x0;
+      float yd = (float)py - y0;
+      if (COUNT) *reads = nr;
+      return __builtin_sqrtf(xd * xd + yd * yd);
+    }
+    float d = (code != RL_IDX8_ESCAPE) ? lut[code] : m.edt[(size_t)ux * H + uy];
+    float st = d * 0.999f;
+    t += (st < 1.0f) ? 1.0f : st;
+  }
+  if (COUNT) *reads = nr;
+  return max_range;
+}
+
+__device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_range) {
+  return fabsf(x0) < 1.0e9f && fabsf(y0) < 1.0e9f && max_range < 1.0e9f;
 }
 
 // OMap::isOccupied (RangeLib.h:204-210): out of bounds reads as free.

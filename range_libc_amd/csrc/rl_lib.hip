@@ -176,12 +176,25 @@ __device__ __forceinline__ float ray_pose(const MapDev& m, const float* __restri
 }
 
 template <int METHOD, bool COUNT>
-__device__ __forceinline__ float cast_one(const MapDev& m, const TrigDev& trig, float max_range,
-                                          float x, float y, float heading, int* cnt) {
+__device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, const rl::Box& box,
+                                          const TrigDev& trig, float max_range, float x, float y,
+                                          float heading, int* cnt) {
   float s, c;
   rl::sincos_libc(heading, trig, &s, &c);
   if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
+  if (m.idx8 && rl::fast_march_ok(x, y, max_range))
+    return rl::rm_march_idx8<COUNT>(m, lut, box, max_range, x, y, c, s, cnt);
   return rl::rm_march<COUNT>(m.edt, m.width, m.height, max_range, x, y, c, s, cnt);
+}
+
+// Copies the distance table into LDS (ray marching over the compact map only).
+// Must be reached by every thread of the block.
+template <int METHOD>
+__device__ __forceinline__ void stage_lut(const MapDev& m, float* sh) {
+  if (METHOD == RL_METHOD_RM && m.idx8) {
+    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) sh[i] = m.lut[i];
+    __syncthreads();
+  }
 }
 
 template <int METHOD, int FRAME, bool COUNT>
@@ -190,12 +203,14 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
                                               const float* __restrict__ angles,
                                               float* __restrict__ outs, int* __restrict__ counts,
                                               int n_rays, int num_angles) {
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rays) return;
   float x, y, h;
   float scale = ray_pose<FRAME>(m, ins, angles, r, num_angles, &x, &y, &h);
   int cnt = 0;
-  float d = cast_one<METHOD, COUNT>(m, trig, max_range, x, y, h, &cnt);
+  float d = cast_one<METHOD, COUNT>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, x, y, h, &cnt);
   if (COUNT)
     counts[r] = cnt;
   else
@@ -207,7 +222,13 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
 // product over beams in beam order, as the reference's loop forms it
 // (RangeLib.h:542-554), so it is bit-identical: the terms are broadcast
 // across the segment with shuffles and multiplied sequentially.
-__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt) {
+__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt, int S) {
+  if (S == 64) {  // one particle per wave: broadcast with scalar lane reads
+    int lo = __double2loint(term), hi = __double2hiint(term);
+    for (int jj = 0; jj < cnt; ++jj)
+      w *= __hiloint2double(__builtin_amdgcn_readlane(hi, jj), __builtin_amdgcn_readlane(lo, jj));
+    return w;
+  }
   for (int jj = 0; jj < cnt; ++jj) w *= __shfl(term, base + jj, 64);
   return w;
 }
@@ -233,7 +254,7 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
       float d = rl::clamp_table(ranges[(size_t)p * M + j] * inv_scale, hi);
       term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = segment_product(w, term, base, min(S, M - c0));
+    w = segment_product(w, term, base, min(S, M - c0), S);
   }
   if (valid && j0 == 0) outs[p] = w;
 }
@@ -252,6 +273,8 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
   const int p = wave * per_wave + seg;
   const bool valid = p < N;
   const float hi = (float)((double)(float)K - 1.0);
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
   float gx = 0.0f, gy = 0.0f, theta = 0.0f;
   if (valid) {  // RangeLib.h:589-597
     rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
@@ -264,14 +287,189 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
     if (valid && j < M) {
       int unused;
       // the range stays in grid units here: no world_scale (RangeLib.h:602-603)
-      float d = cast_one<METHOD, false>(m, trig, max_range, gy, gx, theta - angles[j], &unused);
+      float d = cast_one<METHOD, false>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, gy, gx,
+                                        theta - angles[j], &unused);
       d = rl::clamp_table(d, hi);
       float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
       term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = segment_product(w, term, base, min(S, M - c0));
+    w = segment_product(w, term, base, min(S, M - c0), S);
   }
   if (valid && j0 == 0) weights[p] = w;
+}
+
+// XCD-aware scheduling of particles.  The map is cut into RL_REGIONS
+// rectangles (4 x 2 or 2 x 4, the longer side cut in four); k_region buckets
+// the particles by the region their origin cell lies in.  The particle kernel
+// then runs RL_REGIONS * ceil(N / per_block) workgroups: workgroup b takes the
+// (b / RL_REGIONS)-th chunk of region b % RL_REGIONS and exits at once when
+// that chunk is empty.  Workgroups are dispatched to the XCDs round-robin
+// (b -> XCD b % 8 on MI355X), so each XCD's L2 serves about one eighth of the
+// map; correctness does not depend on that placement.  No atomics.
+#define RL_REGIONS 8
+#define RL_XCD_SCHED_MAX 65536  // larger batches fill the GPU many times over: plain order
+
+struct Sched {
+  int* perm;    // particle indices grouped by region
+  int* count;   // [RL_REGIONS]
+  int* offset;  // [RL_REGIONS]
+};
+
+__device__ __forceinline__ int region_of(const MapDev& m, float x0, float y0) {
+  // x0, y0: grid coordinates of the ray origin (first index, second index)
+  int cx = (x0 >= 0.0f && x0 < (float)m.width) ? (int)x0 : 0;
+  int cy = (y0 >= 0.0f && y0 < (float)m.height) ? (int)y0 : 0;
+  int rx = m.width >= m.height ? 4 : 2;
+  int ry = RL_REGIONS / rx;
+  int ix = (int)(((long long)cx * rx) / m.width);
+  int iy = (int)(((long long)cy * ry) / m.height);
+  return ix * ry + iy;
+}
+
+__device__ __forceinline__ int particle_region(const MapDev& m, const float* __restrict__ ins, int p) {
+  float gx, gy;
+  rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
+  return region_of(m, gy, gx);  // calc_range(y, x, .) marches from (gy, gx)
+}
+
+// One workgroup of 1024 threads buckets N particles.
+__global__ __launch_bounds__(1024) void k_region(MapDev m, const float* __restrict__ ins, int N, Sched s) {
+  __shared__ int cnt[RL_REGIONS], fill[RL_REGIONS];
+  if (threadIdx.x < RL_REGIONS) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < N; p += blockDim.x) atomicAdd(&cnt[particle_region(m, ins, p)], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int r = 0; r < RL_REGIONS; ++r) {
+      s.count[r] = cnt[r];
+      s.offset[r] = run;
+      fill[r] = run;
+      run += cnt[r];
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < N; p += blockDim.x) {
+    int pos = atomicAdd(&fill[particle_region(m, ins, p)], 1);
+    s.perm[pos] = p;
+  }
+}
+
+// All beams of particle p (p < 0: none) on the lanes of one wave segment:
+// numpy_calc_range_angles (RangeLib.h:482-520) and, with SENSOR,
+// eval_sensor_model (RangeLib.h:533-555) on the ranges just written -- each
+// range is read back as the separate call would (range * inv_world_scale), so
+// both outputs are bit-identical to the two calls.  Must be reached by the
+// whole wave (the product broadcasts across lanes).
+template <int METHOD, bool SENSOR>
+__device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut, const TrigDev& trig,
+                                               float max_range, const float* __restrict__ ins,
+                                               const float* __restrict__ angles,
+                                               const float* __restrict__ obs,
+                                               const double* __restrict__ table, int K,
+                                               float* __restrict__ ranges,
+                                               double* __restrict__ weights, int p, int M, int S,
+                                               int j0, int base, uint8_t* box_lds) {
+  const bool valid = p >= 0;
+  const float hi = (float)((double)(float)K - 1.0);
+  float gx = 0.0f, gy = 0.0f, theta = 0.0f;
+  if (valid) {
+    rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
+    theta = -ins[3 * (size_t)p + 2] + m.world.rotation_const;
+  }
+  // one particle per wave: its beams share the LDS box around the origin
+  rl::Box box{nullptr, 0, 0};
+  if (METHOD == RL_METHOD_RM && S == 64 && m.idx8 && m.use_box && box_lds)
+    box = rl::stage_box(m, box_lds, gy, gx, valid && rl::fast_march_ok(gy, gx, max_range));
+  double w = 1.0;
+  for (int c0 = 0; c0 < M; c0 += S) {
+    int j = c0 + j0;
+    double term = 1.0;
+    if (valid && j < M) {
+      int unused;
+      float d = cast_one<METHOD, false>(m, lut, box, trig, max_range, gy, gx, theta - angles[j], &unused) *
+                m.world.scale;
+      ranges[(size_t)p * M + j] = d;
+      if (SENSOR) {
+        float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
+        float dd = rl::clamp_table(d * m.world.inv_scale, hi);
+        term = table[(size_t)(int)r * (size_t)K + (size_t)(int)dd];
+      }
+    }
+    if (SENSOR) w = segment_product(w, term, base, min(S, M - c0), S);
+  }
+  if (SENSOR && valid && j0 == 0) weights[p] = w;
+}
+
+// Particles in input order: wave segment -> particle.
+template <int METHOD, bool SENSOR>
+__global__ __launch_bounds__(256) void k_particles(MapDev m, TrigDev trig, float max_range,
+                                                   const float* __restrict__ ins,
+                                                   const float* __restrict__ angles,
+                                                   const float* __restrict__ obs,
+                                                   const double* __restrict__ table, int K,
+                                                   float* __restrict__ ranges,
+                                                   double* __restrict__ weights, int N, int M, int S) {
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int per_wave = 64 / S;
+  const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
+  int p = wave * per_wave + seg;
+  __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights,
+                                 p < N ? p : -1, M, S, j0, base, boxes + (threadIdx.x >> 6) * RL_BOX_BYTES);
+}
+
+// Particles of one region chunk per workgroup (see Sched).
+template <int METHOD, bool SENSOR>
+__global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, float max_range,
+                                                       const float* __restrict__ ins,
+                                                       const float* __restrict__ angles,
+                                                       const float* __restrict__ obs,
+                                                       const double* __restrict__ table, int K,
+                                                       float* __restrict__ ranges,
+                                                       double* __restrict__ weights, int M, int S,
+                                                       Sched sched) {
+  const int per_wave = 64 / S;
+  const int per_block = (blockDim.x >> 6) * per_wave;
+  const int r = blockIdx.x % RL_REGIONS;
+  const long long first = (long long)(blockIdx.x / RL_REGIONS) * per_block;
+  const int cnt = sched.count[r];
+  if (first >= cnt) return;  // uniform across the workgroup
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
+  const long long idx = first + wib * per_wave + seg;
+  const int p = idx < cnt ? sched.perm[sched.offset[r] + idx] : -1;
+  __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights,
+                                 p, M, S, j0, base, boxes + wib * RL_BOX_BYTES);
+}
+
+// Compact index map from the fp32 distance transform (layout in rl_device.h).
+__global__ void k_build_idx8(const float* __restrict__ edt, const float* __restrict__ lut,
+                             uint8_t* __restrict__ idx8, int W, int H, int nby, size_t nbytes) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbytes) return;
+  size_t b = i >> 6;
+  unsigned o = (unsigned)(i & 63);
+  unsigned px = (unsigned)(b / (size_t)nby) * 8u + (o >> 3);
+  unsigned py = (unsigned)(b % (size_t)nby) * 8u + (o & 7u);
+  unsigned code = RL_IDX8_ESCAPE;
+  if (px < (unsigned)W && py < (unsigned)H) {
+    float d = edt[(size_t)px * H + py];
+    int lo = 0, hi = RL_IDX8_ESCAPE - 1;  // lut[0..254] strictly increasing
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (lut[mid] < d) lo = mid + 1; else hi = mid;
+    }
+    if (lut[lo] == d) code = (unsigned)lo;
+  }
+  idx8[i] = (uint8_t)code;
 }
 
 __global__ void k_debug_sincos(TrigDev trig, const float* __restrict__ x, float* __restrict__ s,
@@ -323,6 +521,8 @@ struct TrigFile {
   double delta_s = 0, delta_c = 0;
   std::vector<uint32_t> off_s, off_c;
   std::vector<uint16_t> keys_s, keys_c;
+  int e_min_s = 0, e_min_c = 0;
+  std::vector<uint8_t> bits_s, bits_c;  // flat exception bitmaps (rl_trig.h)
   std::string libc_version;
 };
 
@@ -378,6 +578,12 @@ const TrigFile* trig_table(std::string* err) {
   }
   g_trig.delta_s = h.delta_sin;
   g_trig.delta_c = h.delta_cos;
+  g_trig.e_min_s = rltrig::table_min_exponent(g_trig.off_s.data(), g_trig.keys_s.data());
+  g_trig.e_min_c = rltrig::table_min_exponent(g_trig.off_c.data(), g_trig.keys_c.data());
+  g_trig.bits_s.assign(rltrig::bitmap_bytes(g_trig.e_min_s), 0);
+  g_trig.bits_c.assign(rltrig::bitmap_bytes(g_trig.e_min_c), 0);
+  rltrig::table_to_bitmap(g_trig.off_s.data(), g_trig.keys_s.data(), g_trig.e_min_s, g_trig.bits_s.data());
+  rltrig::table_to_bitmap(g_trig.off_c.data(), g_trig.keys_c.data(), g_trig.e_min_c, g_trig.bits_c.data());
   h.libc_version[31] = 0;
   g_trig.libc_version = h.libc_version;
   g_trig.loaded = true;
@@ -400,6 +606,9 @@ struct rl_ctx {
   hipStream_t stream = nullptr;
   void* trig_buf = nullptr;
   TrigDev trig{};
+  float* d_lut = nullptr;  // RL_LUT_SIZE tabulated distances (see rl_device.h)
+  int xcd_sched = 0;          // RL_OPT_XCD_SCHED
+  int lds_box = 1;            // RL_OPT_LDS_BOX
   std::mutex mu;
   void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
@@ -428,12 +637,18 @@ struct rl_map {
   int width = 0, height = 0;
   float* d_edt = nullptr;
   uint8_t* d_occ = nullptr;
+  uint8_t* d_idx8 = nullptr;  // compact blocked index map (null when disabled)
+  int* d_sched = nullptr;     // region buckets: count | offset | pad | perm
+  int sched_cap = 0;
   double* d_table = nullptr;
   int table_width = 0;
+  int nby = 0;
   WorldDev world{};
   float edt_ms = 0.0f;
 
-  MapDev dev() const { return MapDev{d_edt, d_occ, width, height, world}; }
+  MapDev dev() const {
+    return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut, width, height, nby, ctx->lds_box, world};
+  }
 };
 
 namespace {
@@ -480,6 +695,9 @@ int launch_cast_f(rl_map* m, int method, float max_range, const float* ins, cons
   return launch_cast_t<RL_METHOD_RM, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
 }
 
+int launch_particles(rl_map* m, int method, bool sensor, float max_range, const float* ins,
+                     const float* angles, const float* obs, float* ranges, double* weights, int N, int M);
+
 int launch_cast(rl_map* m, int method, int frame, bool count, float max_range, const float* ins,
                 const float* angles, float* outs, int* counts, int n_rays, int num_angles) {
   TRY(check_method(method));
@@ -490,6 +708,9 @@ int launch_cast(rl_map* m, int method, int frame, bool count, float max_range, c
   if (frame == RL_FRAME_WORLD)
     return count ? launch_cast_f<1, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
                  : launch_cast_f<1, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  if (frame == RL_FRAME_ANGLES && !count && num_angles > 0)
+    return launch_particles(m, method, false, max_range, ins, angles, nullptr, outs, nullptr,
+                            n_rays / num_angles, num_angles);
   if (frame == RL_FRAME_ANGLES)
     return count ? launch_cast_f<2, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
                  : launch_cast_f<2, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
@@ -523,6 +744,63 @@ int launch_fused(rl_map* m, int method, float max_range, const float* ins, const
                        max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
   HIP_TRY(hipGetLastError());
   return RL_OK;
+}
+
+// Region buckets for XCD-aware scheduling, grown to hold n particles.
+int ensure_sched(rl_map* m, int n) {
+  if (m->sched_cap >= n && m->d_sched) return RL_OK;
+  if (m->d_sched) HIP_TRY(hipFree(m->d_sched));
+  m->d_sched = nullptr;
+  m->sched_cap = 0;
+  size_t cap = (size_t)n + (size_t)n / 4 + 64;
+  if (hipMalloc(&m->d_sched, (cap + 4 * RL_REGIONS) * sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    m->d_sched = nullptr;
+    return fail(RL_ERR_NOMEM, "hipMalloc of the scheduling buffer failed (inside graph capture, "
+                              "make one uncaptured call of this size first)");
+  }
+  m->sched_cap = (int)cap;
+  return RL_OK;
+}
+
+template <int METHOD, bool SENSOR>
+int launch_particles_t(rl_map* m, float max_range, const float* ins, const float* angles,
+                       const float* obs, float* ranges, double* weights, int N, int M) {
+  rl_ctx* c = m->ctx;
+  const int S = segment_size(M);
+  const int K = SENSOR ? m->table_width : 1;
+  const double* table = SENSOR ? m->d_table : nullptr;
+  if (c->xcd_sched && N <= RL_XCD_SCHED_MAX) {
+    TRY(ensure_sched(m, N));
+    Sched sc{m->d_sched + 4 * RL_REGIONS, m->d_sched, m->d_sched + RL_REGIONS};
+    hipLaunchKernelGGL(k_region, dim3(1), dim3(1024), 0, c->stream, m->dev(), ins, N, sc);
+    HIP_TRY(hipGetLastError());
+    // enough chunks for all particles to fall in one region; empty chunks exit
+    long long per_block = 4LL * (64 / S);
+    long long blocks = RL_REGIONS * ((N + per_block - 1) / per_block);
+    hipLaunchKernelGGL((k_particles_xcd<METHOD, SENSOR>), dim3((unsigned)blocks), dim3(256), 0, c->stream,
+                       m->dev(), c->trig, max_range, ins, angles, obs, table, K, ranges, weights, M, S, sc);
+  } else {
+    long long waves = ((long long)N + 64 / S - 1) / (64 / S);
+    hipLaunchKernelGGL((k_particles<METHOD, SENSOR>), dim3(grid_for(waves * 64, 256)), dim3(256), 0,
+                       c->stream, m->dev(), c->trig, max_range, ins, angles, obs, table, K, ranges, weights,
+                       N, M, S);
+  }
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+int launch_particles(rl_map* m, int method, bool sensor, float max_range, const float* ins,
+                     const float* angles, const float* obs, float* ranges, double* weights, int N, int M) {
+  TRY(check_method(method));
+  if (sensor && !m->d_table) return fail(RL_ERR_STATE, "no sensor model set (call set_sensor_model first)");
+  if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  if (N == 0) return RL_OK;
+  if (method == RL_METHOD_BL)
+    return sensor ? launch_particles_t<RL_METHOD_BL, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
+                  : launch_particles_t<RL_METHOD_BL, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
+  return sensor ? launch_particles_t<RL_METHOD_RM, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
+                : launch_particles_t<RL_METHOD_RM, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
 }
 
 // Builds the distance transform of m->d_occ into m->d_edt.
@@ -567,6 +845,24 @@ int build_edt(rl_map* m, unsigned flags) {
       break;
     }
     (void)hipEventElapsedTime(&m->edt_ms, e0, e1);
+    if (!(flags & RL_MAP_NO_COMPACT)) {
+      m->nby = (H + 7) / 8;
+      const size_t nbytes = (size_t)((W + 7) / 8) * (size_t)m->nby * 64;
+      if (hipMalloc(&m->d_idx8, nbytes) != hipSuccess) {
+        (void)hipGetLastError();
+        m->d_idx8 = nullptr;
+        rc = fail(RL_ERR_NOMEM, "device allocation of the %zu-byte compact map failed", nbytes);
+        break;
+      }
+      hipLaunchKernelGGL(k_build_idx8, dim3(grid_for((long long)nbytes, 256)), dim3(256), 0, c->stream,
+                         (const float*)m->d_edt, (const float*)c->d_lut, m->d_idx8, W, H, m->nby, nbytes);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        rc = fail(RL_ERR_HIP, "compact map build: %s", hipGetErrorString(e));
+        break;
+      }
+    }
   } while (0);
   if (g) (void)hipFree(g);
   if (v) (void)hipFree(v);
@@ -609,32 +905,44 @@ int rl_create(int device, rl_ctx** out) {
     return fail(RL_ERR_HIP, "hipStreamCreate failed");
   }
   c->stream = c->own_stream;
-  // One device buffer: off_s | off_c | keys_s | keys_c
-  size_t b_off = tf->off_s.size() * 4, b_ks = tf->keys_s.size() * 2, b_kc = tf->keys_c.size() * 2;
-  size_t total = 2 * b_off + b_ks + b_kc + 64;
+  // One device buffer: sin bitmap | cos bitmap
+  size_t b_s = tf->bits_s.size(), b_c = tf->bits_c.size();
+  size_t total = b_s + b_c + 256;
   if (hipMalloc(&c->trig_buf, total) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->own_stream);
     delete c;
-    return fail(RL_ERR_NOMEM, "hipMalloc of the trig table failed");
+    return fail(RL_ERR_NOMEM, "hipMalloc of the trig exception bitmaps failed");
   }
   char* base = (char*)c->trig_buf;
-  hipError_t e = hipMemcpy(base, tf->off_s.data(), b_off, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(base + b_off, tf->off_c.data(), b_off, hipMemcpyHostToDevice);
-  if (e == hipSuccess && b_ks) e = hipMemcpy(base + 2 * b_off, tf->keys_s.data(), b_ks, hipMemcpyHostToDevice);
-  if (e == hipSuccess && b_kc) e = hipMemcpy(base + 2 * b_off + b_ks, tf->keys_c.data(), b_kc, hipMemcpyHostToDevice);
+  hipError_t e = hipSuccess;
+  if (b_s) e = hipMemcpy(base, tf->bits_s.data(), b_s, hipMemcpyHostToDevice);
+  if (e == hipSuccess && b_c) e = hipMemcpy(base + b_s, tf->bits_c.data(), b_c, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(c->trig_buf);
     (void)hipStreamDestroy(c->own_stream);
     delete c;
     return fail(RL_ERR_HIP, "trig table upload: %s", hipGetErrorString(e));
   }
-  c->trig.off_s = (const uint32_t*)base;
-  c->trig.off_c = (const uint32_t*)(base + b_off);
-  c->trig.key_s = (const uint16_t*)(base + 2 * b_off);
-  c->trig.key_c = (const uint16_t*)(base + 2 * b_off + b_ks);
-  c->trig.delta_s = tf->delta_s;
-  c->trig.delta_c = tf->delta_c;
+  c->trig.sin_exc = rltrig::Bitmap{(const uint8_t*)base, tf->e_min_s, tf->delta_s};
+  c->trig.cos_exc = rltrig::Bitmap{(const uint8_t*)(base + b_s), tf->e_min_c, tf->delta_c};
+  // distance table: sqrtf of the 255 smallest sums of two squares
+  {
+    std::vector<char> is_sum(1024, 0);
+    for (int a = 0; a * a < 1024; ++a)
+      for (int b = a; a * a + b * b < 1024; ++b) is_sum[a * a + b * b] = 1;
+    float lut[RL_LUT_SIZE];
+    int k = 0;
+    for (int n = 0; n < 1024 && k < (int)RL_IDX8_ESCAPE; ++n)
+      if (is_sum[n]) lut[k++] = sqrtf((float)n);
+    for (; k < RL_LUT_SIZE; ++k) lut[k] = -1.0f;  // never matches a distance
+    if (hipMalloc(&c->d_lut, sizeof(lut)) != hipSuccess ||
+        hipMemcpy(c->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      rl_destroy(c);
+      return fail(RL_ERR_HIP, "distance table upload failed");
+    }
+  }
   *out = c;
   return RL_OK;
 }
@@ -646,6 +954,7 @@ int rl_destroy(rl_ctx* c) {
   for (int i = 0; i < 6; ++i)
     if (c->stage[i]) (void)hipFree(c->stage[i]);
   if (c->trig_buf) (void)hipFree(c->trig_buf);
+  if (c->d_lut) (void)hipFree(c->d_lut);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return RL_OK;
@@ -711,6 +1020,8 @@ int rl_map_destroy(rl_map* m) {
   }
   if (m->d_occ) (void)hipFree(m->d_occ);
   if (m->d_edt) (void)hipFree(m->d_edt);
+  if (m->d_idx8) (void)hipFree(m->d_idx8);
+  if (m->d_sched) (void)hipFree(m->d_sched);
   if (m->d_table) (void)hipFree(m->d_table);
   delete m;
   return RL_OK;
@@ -916,6 +1227,29 @@ int rl_calc_range_repeat_angles_eval_sensor_model_dev(rl_map* m, int method, flo
   std::lock_guard<std::mutex> lk(m->ctx->mu);
   TRY(use_device(m->ctx));
   return launch_fused(m, method, max_range, ins, angles, obs, weights, N, M);
+}
+
+int rl_pf_step_dev(rl_map* m, int method, float max_range, const float* ins, const float* angles,
+                   const float* obs, float* ranges, double* weights, int N, int M) {
+  CHECK_ARG(m && ins && angles && obs && ranges && weights, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad num_angles / num_particles");
+  CHECK_ARG((long long)M * N <= 0x7fffffffLL, "batch too large");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  return launch_particles(m, method, true, max_range, ins, angles, obs, ranges, weights, N, M);
+}
+
+int rl_set_option(rl_ctx* c, int option, long long value) {
+  CHECK_ARG(c, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (option == RL_OPT_XCD_SCHED) {
+    c->xcd_sched = value != 0;
+  } else if (option == RL_OPT_LDS_BOX) {
+    c->lds_box = value != 0;
+  } else {
+    return fail(RL_ERR_ARG, "unknown option %d", option);
+  }
+  return RL_OK;
 }
 
 int rl_count_steps_dev(rl_map* m, int method, int frame, float max_range, const float* ins,

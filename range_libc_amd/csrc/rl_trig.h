@@ -19,10 +19,13 @@
 //      the file must be built with contraction off, so host and device produce
 //      identical doubles.
 //   2. Round to float.  If the double lies within `delta` of the midpoint
-//      between the two candidate floats, consult an exception table listing
-//      the inputs for which glibc returns the other candidate.  The table and
+//      between the two candidate floats, consult an exception set listing
+//      the inputs for which glibc returns the other candidate.  The set and
 //      delta are produced by exhaustively enumerating every float x with
-//      |x| <= 64 against the C library at build time (gen_trig_table.cpp).
+//      |x| <= 64 against the C library at build time (gen_trig_table.cpp),
+//      stored compactly (sorted keys per 4096-value bucket) and expanded at
+//      load time into a flat bitmap so that the device probes it with a
+//      single load (exceptions cluster: up to ~350 per bucket).
 //
 // Outside |x| <= 64 the result is the double value rounded to nearest, which
 // agrees with glibc except on roughly 0.1% of inputs.
@@ -192,12 +195,83 @@ RL_HD float round_like_libc(double v, float x, const Table& t) {
   return f;
 }
 
-// cosf(h) and sinf(h) as the C library computes them.
+// cosf(h) and sinf(h) as the C library computes them (bucketed table: a
+// scan of sorted keys; used on the host).
 RL_HD void sincosf_libc(float h, const Table& ts, const Table& tc, float* s, float* c) {
   double sd, cd;
   sincos_d(h, &sd, &cd);
   *s = round_like_libc(sd, h, ts);
   *c = round_like_libc(cd, h, tc);
+}
+
+// The same exception set as a flat bitmap: one bit per float encoding of
+// each sign whose exponent field lies in [e_min, 133] (133 = exponent of
+// 64.0f), so a lookup is a single byte load.  e_min is the smallest exponent
+// field of any listed input.
+#define RLT_EXP_MAX 133
+struct Bitmap {
+  const uint8_t* bits;
+  int e_min;
+  double delta;
+};
+
+RL_HD int bitmap_binades(int e_min) { return RLT_EXP_MAX - e_min + 1; }
+
+RL_HD unsigned long long bitmap_bytes(int e_min) {
+  return (2ull * (unsigned long long)bitmap_binades(e_min)) << 20;  // 2^23 bits per binade
+}
+
+// Bit index of x, or -1 when x is outside the covered encodings.
+RL_HD long long bitmap_index(int e_min, float x) {
+  uint32_t xb = f2u(x);
+  uint32_t ab = xb & 0x7fffffffu;
+  if (ab > RLT_ABS_LIMIT) return -1;
+  int e = (int)(ab >> 23);
+  if (e < e_min) return -1;
+  unsigned long long slot = (unsigned long long)(xb >> 31) * (unsigned long long)bitmap_binades(e_min) +
+                            (unsigned long long)(e - e_min);
+  return (long long)((slot << 23) | (ab & 0x7fffffu));
+}
+
+RL_HD void sincosf_libc_bm(float h, const Bitmap& bs, const Bitmap& bc, float* s, float* c) {
+  double sd, cd;
+  sincos_d(h, &sd, &cd);
+  float os, oc;
+  double ds, dc;
+  float fs = neighbours(sd, &os, &ds);
+  float fc = neighbours(cd, &oc, &dc);
+  long long is = ds <= bs.delta ? bitmap_index(bs.e_min, h) : -1;
+  long long ic = dc <= bc.delta ? bitmap_index(bc.e_min, h) : -1;
+  // both probes are issued before either is consumed
+  uint32_t vs = is >= 0 ? (uint32_t)bs.bits[is >> 3] : 0u;
+  uint32_t vc = ic >= 0 ? (uint32_t)bc.bits[ic >> 3] : 0u;
+  *s = (is >= 0 && ((vs >> (is & 7)) & 1u)) ? os : fs;
+  *c = (ic >= 0 && ((vc >> (ic & 7)) & 1u)) ? oc : fc;
+}
+
+// Host-side helpers (plain host functions under hipcc).
+// Smallest exponent field among the listed inputs of a bucketed table (RLT_EXP_MAX + 1 if none).
+static inline int table_min_exponent(const uint32_t* off, const uint16_t* keys) {
+  int e_min = RLT_EXP_MAX + 1;
+  for (uint32_t b = 0; b < RLT_NBUCKETS; ++b)
+    for (uint32_t i = off[b]; i < off[b + 1]; ++i) {
+      uint32_t ab = ((b & ((1u << RLT_SIGN_SHIFT) - 1u)) << RLT_BUCKET_SHIFT) | keys[i];
+      int e = (int)(ab >> 23);
+      if (e < e_min) e_min = e;
+      break;  // keys ascend within a bucket: the first has the smallest exponent
+    }
+  return e_min;
+}
+
+// Sets the bits of every listed input; `bits` holds bitmap_bytes(e_min) zeroed bytes.
+static inline void table_to_bitmap(const uint32_t* off, const uint16_t* keys, int e_min, uint8_t* bits) {
+  for (uint32_t b = 0; b < RLT_NBUCKETS; ++b)
+    for (uint32_t i = off[b]; i < off[b + 1]; ++i) {
+      uint32_t xb = (((b & ((1u << RLT_SIGN_SHIFT) - 1u)) << RLT_BUCKET_SHIFT) | keys[i]) |
+                    ((b >> RLT_SIGN_SHIFT) << 31);
+      long long idx = bitmap_index(e_min, u2f(xb));
+      if (idx >= 0) bits[idx >> 3] |= (uint8_t)(1u << (idx & 7));
+    }
 }
 
 }  // namespace rltrig

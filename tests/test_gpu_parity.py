@@ -228,3 +228,76 @@ def test_step_counts_match_oracle(corridor_map):
         m.ctx.sync()
         _, want = O.Oracle(corridor_map, 500.0, kind).numpy_calc_range_angles(ins, ang, counts=True)
         assert np.array_equal(cnt.cpu().numpy(), want)
+
+
+# --------------------------------------------------------------------------- compact map, fused step
+
+@pytest.mark.parametrize("case", ["rect512", "corridor2048", "open_sparse"])
+def test_compact_map_matches_fp32_map_and_oracle(case):
+    """The 8-bit index map (with escapes to the fp32 map for distances beyond
+    the table) gives the same ranges as the fp32 map and the oracle."""
+    img = {"rect512": lambda: W.rect_disk_map(512, seed=0),
+           "corridor2048": lambda: W.corridor_map(2048, seed=1),
+           # large empty regions: distances up to hundreds of cells -> escapes
+           "open_sparse": lambda: (np.random.default_rng(3).random((700, 900)) < 2e-5).astype(np.uint8)}[case]()
+    torch = _torch()
+    ctx = rl.default_context(0)
+    ins = W.world_poses(img, 1500, seed=21)
+    ang = W.beam_angles(60)
+    want = O.Oracle(img, 500.0, "rm").numpy_calc_range_angles(ins, ang)
+    d_ins, d_ang = torch.from_numpy(ins).cuda(), torch.from_numpy(ang).cuda()
+    for flags in (0, rl._lib.RL_MAP_NO_COMPACT):
+        m = rl.DeviceMap(ctx, O.xmajor(img), flags)
+        out = torch.empty(1500 * 60, dtype=torch.float32, device="cuda")
+        m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, 500.0, d_ins.data_ptr(), d_ang.data_ptr(),
+                                      out.data_ptr(), 1500, 60)
+        ctx.sync()
+        assert_bits_equal(out.cpu().numpy(), want, f"{case} flags={flags}")
+
+
+@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("bl", 60)])
+def test_pf_step_bit_exact(kind, m, corridor_map):
+    """rl_pf_step_dev == numpy_calc_range_angles then eval_sensor_model."""
+    torch = _torch()
+    ctx = rl.default_context(0)
+    table = W.sensor_table(501)
+    n = 3000
+    ins = W.world_poses(corridor_map, n, seed=22)
+    ang = W.beam_angles(m)
+    orc = O.Oracle(corridor_map, 500.0, kind)
+    orc.set_sensor_model(table)
+    obs = W.noisy_observation(orc.numpy_calc_range_angles(W.world_poses(corridor_map, 1, seed=23), ang), seed=24)
+    want_r = orc.numpy_calc_range_angles(ins, ang)
+    want_w = orc.eval_sensor_model(obs, want_r, m, n)
+    dm = rl.DeviceMap(ctx, O.xmajor(corridor_map))
+    dm.set_sensor_table(table)
+    d = {k: torch.from_numpy(v).cuda() for k, v in (("ins", ins), ("ang", ang), ("obs", obs))}
+    rr = torch.empty(n * m, dtype=torch.float32, device="cuda")
+    ww = torch.empty(n, dtype=torch.float64, device="cuda")
+    method = rl.RL_METHOD_RM if kind == "rm" else rl.RL_METHOD_BL
+    dm.pf_step_dev(method, 500.0, d["ins"].data_ptr(), d["ang"].data_ptr(), d["obs"].data_ptr(),
+                   rr.data_ptr(), ww.data_ptr(), n, m)
+    ctx.sync()
+    assert_bits_equal(rr.cpu().numpy(), want_r, "pf ranges")
+    assert np.array_equal(ww.cpu().numpy(), want_w)
+
+
+@pytest.mark.parametrize("xcd,box", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_scheduling_options(xcd, box, corridor_map):
+    """XCD-aware scheduling and LDS staging change only where data is read from."""
+    torch = _torch()
+    ctx = rl.Context(0)
+    ctx.set_option(rl._lib.RL_OPT_XCD_SCHED, xcd)
+    ctx.set_option(rl._lib.RL_OPT_LDS_BOX, box)
+    ins = W.world_poses(corridor_map, 2500, seed=31)
+    ang = W.beam_angles(45)
+    want = O.Oracle(corridor_map, 500.0, "rm").numpy_calc_range_angles(ins, ang)
+    m = rl.DeviceMap(ctx, O.xmajor(corridor_map))
+    d_ins, d_ang = torch.from_numpy(ins).cuda(), torch.from_numpy(ang).cuda()
+    out = torch.empty(2500 * 45, dtype=torch.float32, device="cuda")
+    for _ in range(2):  # the region buckets are rebuilt by every launch
+        out.fill_(-1)
+        m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, 500.0, d_ins.data_ptr(), d_ang.data_ptr(),
+                                      out.data_ptr(), 2500, 45)
+        ctx.sync()
+        assert_bits_equal(out.cpu().numpy(), want, f"xcd={xcd} box={box}")
