@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
-    ap.add_argument("--box", type=int, default=1, help="stage the map around each particle in LDS (1) or not (0)")
+    ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()

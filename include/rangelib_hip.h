@@ -76,7 +76,7 @@ int rl_sync(rl_ctx* ctx);
  * reference's only knobs are the compile-time CHUNK_SIZE / NUM_THREADS,
  * CMakeLists.txt:13-14). */
 #define RL_OPT_XCD_SCHED 1          /* 1: bucket particles by map region per XCD (default 0) */
-#define RL_OPT_LDS_BOX 2            /* 1 (default): stage 64x64 cells around a particle in LDS */
+#define RL_OPT_LDS_BOX 2            /* 1: stage 64x64 cells around a particle in LDS (default 0) */
 int rl_set_option(rl_ctx* ctx, int option, long long value);
 
 /* --- maps ----------------------------------------------------------------- */

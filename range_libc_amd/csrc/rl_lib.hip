@@ -608,7 +608,7 @@ struct rl_ctx {
   TrigDev trig{};
   float* d_lut = nullptr;  // RL_LUT_SIZE tabulated distances (see rl_device.h)
   int xcd_sched = 0;          // RL_OPT_XCD_SCHED
-  int lds_box = 1;            // RL_OPT_LDS_BOX
+  int lds_box = 0;            // RL_OPT_LDS_BOX
   std::mutex mu;
   void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
