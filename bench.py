@@ -83,6 +83,79 @@ def cpu_baseline(img, ins, angles, obs, budget_s=10.0):
             "sample": f"{n} full steps ({PARTICLES}x{BEAMS} rays + sensor model) in {el:.1f} s"}
 
 
+def bench_uneven64(args):
+    """Secondary workload: 64 maps of uneven size (side log-uniform in
+    [256, 4096], rectangle/disk or corridor generator), 500-8000 particles and
+    30-120 beams over 270 degrees each, max_range 500, per-particle sensor
+    model -- all cast in one autotuned launch, compared with one fused launch
+    per map."""
+    import torch
+
+    import range_libc_amd as rl
+    from range_libc_amd import workloads as W
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    table = W.sensor_table(TABLE)
+    specs = W.uneven_batch_specs(64, seed=4)
+    t0 = time.perf_counter()
+    jobs, keep = [], []
+    total_rays = 0
+    for i, sp in enumerate(specs):
+        img = W.make_map(sp["kind"], sp["side"], sp["map_seed"])
+        n, m = sp["particles"], sp["beams"]
+        dm = rl.DeviceMap(ctx, np.ascontiguousarray(img.T))
+        dm.set_sensor_table(table)
+        ins = torch.from_numpy(W.world_poses(img, n, sp["pose_seed"])).to(dev)
+        ang = torch.from_numpy(W.beam_angles(m)).to(dev)
+        true = torch.from_numpy(W.world_poses(img, 1, sp["pose_seed"] + 1)).to(dev)
+        tr = torch.empty(m, device=dev)
+        dm.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, true.data_ptr(), ang.data_ptr(), tr.data_ptr(), 1, m)
+        stream.synchronize()
+        obs = torch.from_numpy(W.noisy_observation(tr.cpu().numpy(), i, max_range=MAX_RANGE)).to(dev)
+        rng_ = torch.empty(n * m, device=dev)
+        w = torch.empty(n, dtype=torch.float64, device=dev)
+        jobs.append({"map": dm, "max_range": MAX_RANGE, "ins": ins.data_ptr(), "angles": ang.data_ptr(),
+                     "obs": obs.data_ptr(), "ranges": rng_.data_ptr(), "weights": w.data_ptr(),
+                     "num_particles": n, "num_angles": m})
+        keep.append((dm, ins, ang, obs, rng_, w))
+        total_rays += n * m
+    setup_s = time.perf_counter() - t0
+    batch = rl.Batch(ctx)
+    batch.set_jobs(jobs)
+    tuned_ms = batch.autotune(reps=3)
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(50_000_000)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    batch.launch()
+    batch_ms = timed(batch.launch, args.steps)
+
+    def per_map():
+        for j in jobs:
+            j["map"].pf_step_dev(rl.RL_METHOD_RM, MAX_RANGE, j["ins"], j["angles"], j["obs"], j["ranges"],
+                                 j["weights"], j["num_particles"], j["num_angles"])
+
+    per_map()
+    per_map_ms = timed(per_map, max(1, args.steps // 4))
+    print(json.dumps({
+        "metric": "rays/s on 1 GPU (uneven 64-map batch, one launch)", "value": total_rays / (batch_ms * 1e-3),
+        "unit": "rays/s", "ms_per_step": batch_ms, "per_map_launches_ms": per_map_ms,
+        "speedup_vs_per_map": per_map_ms / batch_ms, "autotuned": batch.config(), "autotune_best_ms": tuned_ms,
+        "rays": total_rays, "maps": len(jobs), "setup_s": setup_s, "steps": args.steps,
+        "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +165,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
+    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64"],
+                    help="pf_step (the headline particle-filter step) or uneven64 (multi-map batch)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -104,6 +179,8 @@ def main():
     import range_libc_amd as rl
 
     dev = torch.device("cuda", 0)
+    if args.workload == "uneven64":
+        return bench_uneven64(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)

@@ -151,6 +151,42 @@ int rl_pf_step_dev(rl_map* map, int method, float max_range, const float* d_ins,
                    const float* d_angles, const float* d_obs, float* d_ranges, double* d_weights,
                    int num_particles, int num_angles);
 
+/* --- multi-map batches ----------------------------------------------------------
+ * Many maps of uneven size, each with its own particles, beam count and
+ * (optional) sensor model, cast in one launch.  No counterpart in the
+ * reference, whose batches are one map per call in CHUNK_SIZE pieces
+ * (RangeLib.h:819-874).  Each job is numpy_calc_range_angles (RangeLib.h:482-520)
+ * of its map and, when d_obs is set, eval_sensor_model (RangeLib.h:533-555) of
+ * the ranges; outputs are bit-identical to those calls.  All pointers are
+ * device pointers; maps and buffers must outlive the launches. */
+typedef struct {
+  rl_map* map;
+  float max_range;
+  const float* d_ins;     /* num_particles x 3 poses (world frame) */
+  const float* d_angles;  /* num_angles */
+  const float* d_obs;     /* num_angles observed ranges, or NULL: no sensor model */
+  float* d_ranges;        /* num_particles * num_angles */
+  double* d_weights;      /* num_particles (when d_obs is set) */
+  int num_particles;
+  int num_angles;
+} rl_batch_job;
+
+typedef struct rl_batch rl_batch;
+int rl_batch_create(rl_ctx* ctx, int method, rl_batch** out);
+int rl_batch_destroy(rl_batch* batch);
+/* Uploads the job descriptors and measures each map's mean reads per ray on a
+ * sample of its rays (synchronous; not for graph capture). */
+int rl_batch_set_jobs(rl_batch* batch, const rl_batch_job* jobs, int n_jobs);
+/* Times every launch configuration (lanes per particle 64/32/16, LDS box,
+ * work order, grid size) `reps` times on the current jobs and keeps the
+ * fastest; *best_ms receives its time per launch. */
+int rl_batch_autotune(rl_batch* batch, int reps, float* best_ms);
+int rl_batch_set_config(rl_batch* batch, int rays_per_wave, int lds_box, int order, int grid);
+int rl_batch_get_config(rl_batch* batch, int* rays_per_wave, int* lds_box, int* order, int* grid);
+/* One launch over all jobs on the context's stream: no allocation, no copy,
+ * so it may be captured in a HIP graph. */
+int rl_batch_launch(rl_batch* batch);
+
 /* --- instrumentation (device pointers) -------------------------------------- */
 /* Per-ray work counts of a batch: distance-transform reads (RM) or cells
  * visited (BL).  `frame` is RL_FRAME_*; `angles`/`num_angles` are used by

@@ -60,6 +60,13 @@ SIGNATURES = {
                                                           _vp, C.c_int, C.c_int],
     "rl_pf_step_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int],
     "rl_count_steps_dev": [_vp, C.c_int, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],
+    "rl_batch_create": [_vp, C.c_int, C.POINTER(_vp)],
+    "rl_batch_destroy": [_vp],
+    "rl_batch_set_jobs": [_vp, _vp, C.c_int],
+    "rl_batch_autotune": [_vp, C.c_int, _fp],
+    "rl_batch_set_config": [_vp, C.c_int, C.c_int, C.c_int, C.c_int],
+    "rl_batch_get_config": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "rl_batch_launch": [_vp],
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
 }
@@ -269,3 +276,57 @@ class DeviceMap:
     def count_steps_dev(self, method, frame, max_range, ins, angles, counts, n, m=0):
         check(lib().rl_count_steps_dev(self.handle, method, frame, max_range, _vp(ins), _vp(angles or 0),
                                        _vp(counts), n, m))
+
+
+class BatchJob(C.Structure):
+    """rl_batch_job: one map's particle-filter work inside a multi-map batch."""
+    _fields_ = [("map", _vp), ("max_range", C.c_float), ("d_ins", _vp), ("d_angles", _vp), ("d_obs", _vp),
+                ("d_ranges", _vp), ("d_weights", _vp), ("num_particles", C.c_int), ("num_angles", C.c_int)]
+
+
+class Batch:
+    """Many maps cast in one launch (rl_batch).  Jobs hold device addresses."""
+
+    def __init__(self, ctx: Context, method: int = RL_METHOD_RM):
+        h = C.c_void_p()
+        check(lib().rl_batch_create(ctx.handle, method, C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rl_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_jobs(self, jobs):
+        """jobs: list of dicts with keys map (DeviceMap), max_range, ins, angles, obs (or 0),
+        ranges, weights (or 0), num_particles, num_angles (device addresses as ints)."""
+        arr = (BatchJob * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = BatchJob(j["map"].handle, j["max_range"], j["ins"], j["angles"], j.get("obs") or None,
+                              j["ranges"], j.get("weights") or None, j["num_particles"], j["num_angles"])
+        self._keep = [j["map"] for j in jobs]
+        check(lib().rl_batch_set_jobs(self.handle, C.cast(arr, _vp), len(jobs)))
+
+    def autotune(self, reps: int = 3) -> float:
+        ms = C.c_float()
+        check(lib().rl_batch_autotune(self.handle, reps, C.byref(ms)))
+        return ms.value
+
+    def set_config(self, rays_per_wave=64, lds_box=0, order=1, grid=0):
+        check(lib().rl_batch_set_config(self.handle, rays_per_wave, lds_box, order, grid))
+
+    def config(self):
+        v = [C.c_int() for _ in range(4)]
+        check(lib().rl_batch_get_config(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(("rays_per_wave", "lds_box", "order", "grid"), (x.value for x in v)))
+
+    def launch(self):
+        check(lib().rl_batch_launch(self.handle))

@@ -1,0 +1,320 @@
+// Multi-map batches: many maps of uneven size, each with its own particles,
+// beam count and sensor model, cast in one persistent launch.
+//
+// Included at the end of rl_lib.hip (uses its internal types and
+// particle_beams).  The work list holds one item per (map, particle chunk); a
+// chunk is the particles one 256-thread workgroup casts together (4 waves x
+// 64 / S particles, S = lanes per particle).  k_batch runs a fixed grid whose
+// workgroups stride over the list, so long items placed first (order 1:
+// estimated cost, descending) finish while later short items fill the gaps.
+//
+// Tuned per batch by rl_batch_autotune: lanes per particle S (16, 32 or 64,
+// never above the beam count's power of two), LDS staging of the 64x64 cells
+// around each particle (S = 64 only), item order, and grid size.  The
+// estimated cost of an item is its ray count times the mean number of
+// distance-map reads per ray of its map, measured on a sample of the map's
+// own rays when the jobs are set.
+
+struct BatchMap {
+  MapDev m;
+  float max_range;
+  const float* ins;
+  const float* angles;
+  const float* obs;
+  const double* table;
+  int K;
+  float* ranges;
+  double* weights;
+  int N, M, S;
+  int sensor;
+};
+
+struct BatchItem {
+  int map;
+  int first;  // first particle of the chunk
+};
+
+template <int METHOD>
+__global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps,
+                                               const BatchItem* __restrict__ items, int n_items,
+                                               TrigDev trig, int use_box) {
+  __shared__ float lut[RL_LUT_SIZE];
+  __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  if (METHOD == RL_METHOD_RM) {  // one distance table per context: the same for every map
+    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = maps[0].m.lut[i];
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const int wib = threadIdx.x >> 6;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const BatchItem item = items[it];
+    const BatchMap& bm = maps[item.map];
+    MapDev m = bm.m;
+    m.use_box = use_box;
+    const int S = bm.S;
+    const int per_wave = 64 / S;
+    const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
+    const int p = item.first + wib * per_wave + seg;
+    const int pp = p < bm.N ? p : -1;
+    if (bm.sensor)
+      particle_beams<METHOD, true>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
+                                   bm.ranges, bm.weights, pp, bm.M, S, j0, base,
+                                   boxes + wib * RL_BOX_BYTES);
+    else
+      particle_beams<METHOD, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
+                                    bm.ranges, bm.weights, pp, bm.M, S, j0, base,
+                                    boxes + wib * RL_BOX_BYTES);
+  }
+}
+
+struct rl_batch {
+  rl_ctx* ctx = nullptr;
+  int method = RL_METHOD_RM;
+  std::vector<rl_batch_job> jobs;
+  std::vector<double> reads_per_ray;  // measured per job
+  // launch configuration
+  int rays_per_wave = 64;  // upper bound on S
+  int lds_box = 0;
+  int order = 1;           // 0: job order, 1: estimated cost descending
+  int grid = 0;            // 0: one workgroup per item
+  // device descriptors of the current configuration
+  BatchMap* d_maps = nullptr;
+  BatchItem* d_items = nullptr;
+  int n_items = 0;
+  size_t cap_maps = 0, cap_items = 0;
+};
+
+namespace {
+
+int batch_upload(rl_batch* b) {
+  rl_ctx* c = b->ctx;
+  const int nj = (int)b->jobs.size();
+  std::vector<BatchMap> maps(nj);
+  std::vector<std::pair<double, BatchItem>> items;
+  for (int i = 0; i < nj; ++i) {
+    const rl_batch_job& j = b->jobs[i];
+    BatchMap& bm = maps[i];
+    bm.m = j.map->dev();
+    bm.max_range = j.max_range;
+    bm.ins = j.d_ins;
+    bm.angles = j.d_angles;
+    bm.obs = j.d_obs;
+    bm.sensor = j.d_obs != nullptr;
+    bm.table = bm.sensor ? j.map->d_table : nullptr;
+    bm.K = bm.sensor ? j.map->table_width : 1;
+    bm.ranges = j.d_ranges;
+    bm.weights = j.d_weights;
+    bm.N = j.num_particles;
+    bm.M = j.num_angles;
+    int S = segment_size(j.num_angles);
+    if (S > b->rays_per_wave) S = b->rays_per_wave;
+    bm.S = S;
+    const int chunk = 4 * (64 / S);
+    const double ray_cost = (double)j.num_angles * (1.0 + b->reads_per_ray[i]);
+    for (int f = 0; f < j.num_particles; f += chunk) {
+      int cnt = j.num_particles - f < chunk ? j.num_particles - f : chunk;
+      items.push_back({cnt * ray_cost, BatchItem{i, f}});
+    }
+  }
+  if (b->order == 1)
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<double, BatchItem>& a, const std::pair<double, BatchItem>& z) {
+                       return a.first > z.first;
+                     });
+  std::vector<BatchItem> flat(items.size());
+  for (size_t k = 0; k < items.size(); ++k) flat[k] = items[k].second;
+  if (b->cap_maps < maps.size()) {
+    if (b->d_maps) HIP_TRY(hipFree(b->d_maps));
+    b->d_maps = nullptr;
+    if (hipMalloc(&b->d_maps, maps.size() * sizeof(BatchMap)) != hipSuccess) {
+      (void)hipGetLastError();
+      b->cap_maps = 0;
+      return fail(RL_ERR_NOMEM, "batch descriptor allocation failed");
+    }
+    b->cap_maps = maps.size();
+  }
+  if (b->cap_items < flat.size()) {
+    if (b->d_items) HIP_TRY(hipFree(b->d_items));
+    b->d_items = nullptr;
+    if (hipMalloc(&b->d_items, flat.size() * sizeof(BatchItem)) != hipSuccess) {
+      (void)hipGetLastError();
+      b->cap_items = 0;
+      return fail(RL_ERR_NOMEM, "batch work-list allocation failed");
+    }
+    b->cap_items = flat.size();
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_maps, maps.data(), maps.size() * sizeof(BatchMap), hipMemcpyHostToDevice,
+                         c->stream));
+  if (!flat.empty())
+    HIP_TRY(hipMemcpyAsync(b->d_items, flat.data(), flat.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  b->n_items = (int)flat.size();
+  return RL_OK;
+}
+
+int batch_launch_locked(rl_batch* b) {
+  if (b->n_items == 0) return RL_OK;
+  rl_ctx* c = b->ctx;
+  int grid = b->grid > 0 && b->grid < b->n_items ? b->grid : b->n_items;
+  if (b->method == RL_METHOD_BL)
+    hipLaunchKernelGGL(k_batch<RL_METHOD_BL>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
+                       b->n_items, c->trig, b->lds_box);
+  else
+    hipLaunchKernelGGL(k_batch<RL_METHOD_RM>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
+                       b->n_items, c->trig, b->lds_box);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+// Mean distance-map reads per ray of a job, on up to 64 of its particles.
+int batch_probe_reads(rl_batch* b, int i, double* out) {
+  const rl_batch_job& j = b->jobs[i];
+  *out = 1.0;
+  if (b->method != RL_METHOD_RM || j.num_particles == 0) return RL_OK;
+  int np = j.num_particles < 64 ? j.num_particles : 64;
+  size_t n = (size_t)np * j.num_angles;
+  void* d = nullptr;
+  TRY(b->ctx->staging(5, n * sizeof(int), &d));
+  TRY(launch_cast(j.map, b->method, RL_FRAME_ANGLES, true, j.max_range, j.d_ins, j.d_angles, nullptr,
+                  (int*)d, (int)n, j.num_angles));
+  std::vector<int> h(n);
+  HIP_TRY(hipMemcpyAsync(h.data(), d, n * sizeof(int), hipMemcpyDeviceToHost, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  double s = 0;
+  for (int v : h) s += v;
+  *out = s / (double)n;
+  return RL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rl_batch_create(rl_ctx* c, int method, rl_batch** out) {
+  CHECK_ARG(c && out, "null argument");
+  TRY(check_method(method));
+  rl_batch* b = new rl_batch;
+  b->ctx = c;
+  b->method = method;
+  *out = b;
+  return RL_OK;
+}
+
+int rl_batch_destroy(rl_batch* b) {
+  if (!b) return RL_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->d_maps) (void)hipFree(b->d_maps);
+  if (b->d_items) (void)hipFree(b->d_items);
+  delete b;
+  return RL_OK;
+}
+
+int rl_batch_set_jobs(rl_batch* b, const rl_batch_job* jobs, int n_jobs) {
+  CHECK_ARG(b && (jobs || n_jobs == 0) && n_jobs >= 0, "bad argument");
+  for (int i = 0; i < n_jobs; ++i) {
+    const rl_batch_job& j = jobs[i];
+    if (!j.map || !j.d_ins || !j.d_angles || !j.d_ranges)
+      return fail(RL_ERR_ARG, "job %d: null map, poses, angles or ranges", i);
+    if (j.map->ctx != b->ctx) return fail(RL_ERR_ARG, "job %d: map belongs to another context", i);
+    if (j.num_particles < 0 || j.num_angles < 1) return fail(RL_ERR_ARG, "job %d: bad sizes", i);
+    if ((long long)j.num_particles * j.num_angles > 0x7fffffffLL) return fail(RL_ERR_ARG, "job %d: too many rays", i);
+    if (j.d_obs && (!j.d_weights || !j.map->d_table))
+      return fail(RL_ERR_ARG, "job %d: the sensor model needs weights and a sensor table", i);
+    if (b->method == RL_METHOD_RM && !j.map->d_edt) return fail(RL_ERR_STATE, "job %d: map has no distance transform", i);
+  }
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  TRY(use_device(b->ctx));
+  b->jobs.assign(jobs, jobs + n_jobs);
+  b->reads_per_ray.assign(n_jobs, 1.0);
+  for (int i = 0; i < n_jobs; ++i) TRY(batch_probe_reads(b, i, &b->reads_per_ray[i]));
+  return batch_upload(b);
+}
+
+int rl_batch_set_config(rl_batch* b, int rays_per_wave, int lds_box, int order, int grid) {
+  CHECK_ARG(b, "batch is NULL");
+  CHECK_ARG(rays_per_wave == 16 || rays_per_wave == 32 || rays_per_wave == 64, "rays_per_wave must be 16, 32 or 64");
+  CHECK_ARG(order == 0 || order == 1, "order must be 0 or 1");
+  CHECK_ARG(grid >= 0, "grid must be >= 0");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  TRY(use_device(b->ctx));
+  b->rays_per_wave = rays_per_wave;
+  b->lds_box = lds_box != 0;
+  b->order = order;
+  b->grid = grid;
+  return batch_upload(b);
+}
+
+int rl_batch_get_config(rl_batch* b, int* rays_per_wave, int* lds_box, int* order, int* grid) {
+  CHECK_ARG(b, "batch is NULL");
+  if (rays_per_wave) *rays_per_wave = b->rays_per_wave;
+  if (lds_box) *lds_box = b->lds_box;
+  if (order) *order = b->order;
+  if (grid) *grid = b->grid;
+  return RL_OK;
+}
+
+int rl_batch_launch(rl_batch* b) {
+  CHECK_ARG(b, "batch is NULL");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  TRY(use_device(b->ctx));
+  return batch_launch_locked(b);
+}
+
+int rl_batch_autotune(rl_batch* b, int reps, float* best_ms) {
+  CHECK_ARG(b && reps >= 1, "bad argument");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  TRY(use_device(b->ctx));
+  rl_ctx* c = b->ctx;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int cu = 256;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cu = prop.multiProcessorCount;
+  }
+  const int waves[3] = {64, 32, 16};
+  const int grids[3] = {0, 8 * cu, 4 * cu};
+  float best = 1e30f;
+  int bw = b->rays_per_wave, bb = b->lds_box, bo = b->order, bg = b->grid;
+  int rc = RL_OK;
+  for (int w : waves) {
+    for (int box = 0; box < 2 && rc == RL_OK; ++box) {
+      if (box && w != 64) continue;  // the LDS box needs one particle per wave
+      for (int order = 0; order < 2 && rc == RL_OK; ++order) {
+        for (int g : grids) {
+          b->rays_per_wave = w;
+          b->lds_box = box;
+          b->order = order;
+          b->grid = g;
+          if ((rc = batch_upload(b)) != RL_OK) break;
+          if ((rc = batch_launch_locked(b)) != RL_OK) break;  // warm-up
+          if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(RL_ERR_HIP, "event record"); break; }
+          for (int r = 0; r < reps && rc == RL_OK; ++r) rc = batch_launch_locked(b);
+          if (rc != RL_OK) break;
+          if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(RL_ERR_HIP, "event synchronize");
+            break;
+          }
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          ms /= reps;
+          if (ms < best) {
+            best = ms;
+            bw = w, bb = box, bo = order, bg = g;
+          }
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != RL_OK) return rc;
+  b->rays_per_wave = bw, b->lds_box = bb, b->order = bo, b->grid = bg;
+  if (best_ms) *best_ms = best;
+  return batch_upload(b);
+}
+
+}  // extern "C"

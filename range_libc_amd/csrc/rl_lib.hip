@@ -873,6 +873,8 @@ int build_edt(rl_map* m, unsigned flags) {
 }
 
 int use_device(rl_ctx* c) {
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur == c->device) return RL_OK;
   HIP_TRY(hipSetDevice(c->device));
   return RL_OK;
 }
@@ -1279,3 +1281,7 @@ int rl_debug_sqrtf_dev(rl_ctx* c, const float* x, float* out, int n) {
 }
 
 }  // extern "C"
+
+// Multi-map batches (uses the types above).
+#include <algorithm>
+#include "rl_batch_impl.h"

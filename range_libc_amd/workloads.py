@@ -56,14 +56,14 @@ def rect_disk_map(size: int = 512, n_rect: int = 25, n_disk: int = 15, seed: int
         x0 = int(rng.integers(0, max(1, size - w)))
         y0 = int(rng.integers(0, max(1, size - h)))
         img[y0:y0 + h, x0:x0 + w] = 1
-    yy, xx = np.mgrid[0:size, 0:size]
     for _ in range(n_disk):
         r = float(rng.uniform(rmin, rmax))
         cx = float(rng.uniform(0, size))
         cy = float(rng.uniform(0, size))
         x_lo, x_hi = max(0, int(cx - r) - 1), min(size, int(cx + r) + 2)
         y_lo, y_hi = max(0, int(cy - r) - 1), min(size, int(cy + r) + 2)
-        sub = (xx[y_lo:y_hi, x_lo:x_hi] + 0.5 - cx) ** 2 + (yy[y_lo:y_hi, x_lo:x_hi] + 0.5 - cy) ** 2
+        yy, xx = np.ogrid[y_lo:y_hi, x_lo:x_hi]
+        sub = (xx + 0.5 - cx) ** 2 + (yy + 0.5 - cy) ** 2
         img[y_lo:y_hi, x_lo:x_hi][sub <= r * r] = 1
     return img
 
