@@ -187,6 +187,47 @@ int rl_batch_get_config(rl_batch* batch, int* rays_per_wave, int* lds_box, int* 
  * so it may be captured in a HIP graph. */
 int rl_batch_launch(rl_batch* batch);
 
+/* --- learned range surrogate ------------------------------------------------- */
+/* No counterpart in the reference.  A per-map Fourier-feature MLP (64 features
+ * from 32 random frequencies of (x0/W, y0/H, sin h, cos h), three 256-wide ReLU
+ * layers, two outputs: range / max_range and log of an error bound in cells),
+ * bf16 weights with fp32 accumulation on the matrix cores, trained on the GPU on
+ * rays cast by the exact marcher.  Inference is a checked start point for ray
+ * marching: a ray whose predicted bound is <= bound_max cells starts at
+ * max(0, range - 2) if that cell is free and must hit by range + 2 + bound;
+ * every other ray is marched from its origin.  Results approximate the exact
+ * march (a thin wall before the start point is missed); the error is reported,
+ * not bounded.  A surrogate belongs to one map and must not outlive it. */
+typedef struct rl_surrogate rl_surrogate;
+int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out);
+int rl_surrogate_destroy(rl_surrogate* s);
+/* Number of parameters (returned directly, not a status). */
+int rl_surrogate_num_params(void);
+/* Flat fp32 parameters W1 (256x64), b1, W2 (256x256), b2, W3, b3, W4 (2x256), b4
+ * (row-major [out][in]) and the 32 x 4 frequencies; host pointers, either may be NULL. */
+int rl_surrogate_get_params(rl_surrogate* s, float* params, float* freq);
+int rl_surrogate_set_params(rl_surrogate* s, const float* params, const float* freq);
+/* Network outputs (n x 2) for n grid-frame rays (x0, y0, sin h, cos h); host pointers. */
+int rl_surrogate_forward(rl_surrogate* s, const float* rays4, int n, float* out);
+/* Mean loss over n rays with target ranges (cells) and its gradient (num_params
+ * floats); `features` (optional, n x 64 bf16 bit patterns) receives the network's input. */
+int rl_surrogate_gradient(rl_surrogate* s, float max_range, const float* rays4, const float* targets, int n,
+                          float* loss, float* grad, uint16_t* features);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the gradient of the last
+ * rl_surrogate_gradient call, e.g. to train on caller-supplied rays. */
+int rl_surrogate_step(rl_surrogate* s, float lr);
+/* n_pairs / batch Adam steps on uniform free-space rays marched on the GPU;
+ * loss_curve (host, optional) gets one loss per step.  Synchronous. */
+int rl_surrogate_train(rl_surrogate* s, float max_range, long long n_pairs, int batch, float lr, unsigned seed,
+                       float* loss_curve);
+/* numpy_calc_range_angles through the checked surrogate (device pointers,
+ * stream-ordered).  stats (host, 4 counters, optional; synchronises): rays
+ * marched from the origin because the bound exceeded bound_max, because the
+ * start check failed, rays finished from the start point, distance-map reads. */
+int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const float* d_ins,
+                                       const float* d_angles, float* d_ranges, int num_particles,
+                                       int num_angles, float bound_max, unsigned* stats);
+
 /* --- instrumentation (device pointers) -------------------------------------- */
 /* Per-ray work counts of a batch: distance-transform reads (RM) or cells
  * visited (BL).  `frame` is RL_FRAME_*; `angles`/`num_angles` are used by
@@ -197,6 +238,9 @@ int rl_count_steps_dev(rl_map* map, int method, int frame, float max_range, cons
 int rl_debug_sincosf_dev(rl_ctx* ctx, const float* d_x, float* d_sin, float* d_cos, int n);
 /* Device sqrtf as the kernels use it. */
 int rl_debug_sqrtf_dev(rl_ctx* ctx, const float* d_x, float* d_out, int n);
+/* C = A * B through the surrogate's bf16 matrix-core GEMM (host row-major
+ * matrices of bf16 bit patterns; fp32 result). */
+int rl_debug_gemm_bf16(rl_ctx* ctx, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K);
 
 #ifdef __cplusplus
 }

@@ -19,11 +19,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL, RL_METHOD_RM, Batch,
-                   Context, DeviceMap, RangeLibError, default_context)
+                   Context, DeviceMap, RangeLibError, Surrogate, default_context)
 
 __all__ = [
     "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyNull",
-    "RangeLibError", "Context", "DeviceMap", "Batch", "default_context",
+    "RangeLibError", "Context", "DeviceMap", "Batch", "Surrogate", "default_context",
     "USE_CACHED_TRIG", "USE_ALTERNATE_MOD", "USE_CACHED_CONSTANTS", "USE_FAST_ROUND", "NO_INLINE",
     "USE_LRU_CACHE", "LRU_CACHE_SIZE", "SHOULD_USE_CUDA",
 ]

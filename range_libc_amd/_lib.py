@@ -69,6 +69,18 @@ SIGNATURES = {
     "rl_batch_launch": [_vp],
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
+    "rl_debug_gemm_bf16": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
+    "rl_surrogate_create": [_vp, C.c_uint, C.POINTER(_vp)],
+    "rl_surrogate_destroy": [_vp],
+    "rl_surrogate_num_params": [],
+    "rl_surrogate_get_params": [_vp, _vp, _vp],
+    "rl_surrogate_set_params": [_vp, _vp, _vp],
+    "rl_surrogate_forward": [_vp, _vp, C.c_int, _vp],
+    "rl_surrogate_gradient": [_vp, C.c_float, _vp, _vp, C.c_int, _fp, _vp, _vp],
+    "rl_surrogate_step": [_vp, C.c_float],
+    "rl_surrogate_train": [_vp, C.c_float, C.c_longlong, C.c_int, C.c_float, C.c_uint, _vp],
+    "rl_surrogate_calc_range_angles_dev": [_vp, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
+                                           C.POINTER(C.c_uint)],
 }
 
 
@@ -330,3 +342,100 @@ class Batch:
 
     def launch(self):
         check(lib().rl_batch_launch(self.handle))
+
+
+def debug_gemm_bf16(ctx: Context, a_bits: np.ndarray, b_bits: np.ndarray) -> np.ndarray:
+    """A (M x K) @ B (K x N) of bf16 bit patterns (uint16) on the matrix cores; fp32 result."""
+    a = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    b = np.ascontiguousarray(b_bits, dtype=np.uint16)
+    M, K = a.shape
+    K2, N = b.shape
+    if K != K2:
+        raise ValueError("inner dimensions differ")
+    c = np.empty((M, N), dtype=np.float32)
+    check(lib().rl_debug_gemm_bf16(ctx.handle, ptr(a), ptr(b), ptr(c), M, N, K))
+    return c
+
+
+class Surrogate:
+    """Learned per-map range surrogate (rl_surrogate_*): train on the GPU, then
+    cast particle batches through the checked start point."""
+
+    NUM_FREQ = 32
+
+    def __init__(self, dmap: "DeviceMap", seed: int = 0):
+        h = C.c_void_p()
+        check(lib().rl_surrogate_create(dmap.handle, seed, C.byref(h)))
+        self.handle = h
+        self.map = dmap
+        self.num_params = int(lib().rl_surrogate_num_params())
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rl_surrogate_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.empty(self.num_params, np.float32)
+        f = np.empty((self.NUM_FREQ, 4), np.float32)
+        check(lib().rl_surrogate_get_params(self.handle, ptr(p), ptr(f)))
+        return p, f
+
+    def set_params(self, params=None, freq=None):
+        pp = None if params is None else np.ascontiguousarray(params, np.float32)
+        ff = None if freq is None else np.ascontiguousarray(freq, np.float32)
+        if pp is not None and pp.size != self.num_params:
+            raise ValueError("wrong parameter count")
+        if ff is not None and ff.size != self.NUM_FREQ * 4:
+            raise ValueError("wrong frequency count")
+        check(lib().rl_surrogate_set_params(self.handle, None if pp is None else ptr(pp),
+                                            None if ff is None else ptr(ff)))
+
+    def forward(self, rays4: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(rays4, np.float32).reshape(-1, 4)
+        out = np.empty((r.shape[0], 2), np.float32)
+        check(lib().rl_surrogate_forward(self.handle, ptr(r), r.shape[0], ptr(out)))
+        return out
+
+    def gradient(self, max_range: float, rays4: np.ndarray, targets: np.ndarray):
+        """(loss, grad, features): features are the bf16 bit patterns the network saw."""
+        r = np.ascontiguousarray(rays4, np.float32).reshape(-1, 4)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1)
+        n = r.shape[0]
+        if t.size != n:
+            raise ValueError("one target per ray")
+        g = np.empty(self.num_params, np.float32)
+        feat = np.empty((n, 64), np.uint16)
+        loss = C.c_float()
+        check(lib().rl_surrogate_gradient(self.handle, max_range, ptr(r), ptr(t), n, C.byref(loss), ptr(g),
+                                          ptr(feat)))
+        return loss.value, g, feat
+
+    def step(self, lr: float = 1e-3):
+        """One Adam step with the gradient of the last ``gradient`` call."""
+        check(lib().rl_surrogate_step(self.handle, lr))
+
+    def train(self, max_range: float, n_pairs: int, batch: int = 65536, lr: float = 1e-3, seed: int = 1,
+              curve: bool = True):
+        steps = n_pairs // batch
+        c = np.zeros(max(steps, 1), np.float32) if curve else None
+        check(lib().rl_surrogate_train(self.handle, max_range, n_pairs, batch, lr, seed,
+                                       None if c is None else ptr(c)))
+        return c[:steps] if c is not None else None
+
+    def calc_range_angles_dev(self, max_range, d_ins, d_angles, d_ranges, num_particles, num_angles,
+                              bound_max=0.5, stats=True):
+        st = (C.c_uint * 4)() if stats else None
+        check(lib().rl_surrogate_calc_range_angles_dev(self.handle, max_range, _vp(d_ins), _vp(d_angles),
+                                                       _vp(d_ranges), num_particles, num_angles, bound_max,
+                                                       None if st is None else C.cast(st, C.POINTER(C.c_uint))))
+        if st is None:
+            return None
+        return dict(zip(("fallback_bound", "fallback_check", "from_start", "reads"), (int(v) for v in st)))
+

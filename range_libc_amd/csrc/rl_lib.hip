@@ -1285,3 +1285,8 @@ int rl_debug_sqrtf_dev(rl_ctx* c, const float* x, float* out, int n) {
 // Multi-map batches (uses the types above).
 #include <algorithm>
 #include "rl_batch_impl.h"
+
+// Learned range surrogate (uses the types above).
+#include <cmath>
+#include <random>
+#include "rl_surrogate_impl.h"

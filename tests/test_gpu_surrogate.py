@@ -1,0 +1,205 @@
+"""Learned range surrogate on the GPU, checked against the float64 oracle
+(oracle/surrogate.py): the matrix-core GEMM, features, loss and gradient,
+the Adam step, training, and checked inference against the exact marcher."""
+import math
+
+import numpy as np
+import pytest
+
+import range_libc_amd as rl
+from oracle import oracle as O
+from oracle import surrogate as S
+from range_libc_amd import workloads as W
+from tests.conftest import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU test needs a device"
+    return torch
+
+
+def _rand_bf16(rng, shape, scale=2.0):
+    return S.bf16_bits(rng.uniform(-scale, scale, shape).astype(np.float32))
+
+
+@pytest.mark.parametrize("mnk", [(16, 16, 16), (64, 64, 32), (100, 70, 50), (1, 1, 1), (300, 257, 1000),
+                                 (64, 256, 4096), (2, 256, 3000)])
+def test_gemm_bf16_matches_cpu(ctx, mnk):
+    M, N, K = mnk
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    a, b = _rand_bf16(rng, (M, K)), _rand_bf16(rng, (K, N))
+    c = rl._lib.debug_gemm_bf16(ctx, a, b)
+    A, B = S.bf16_to_f64(a), S.bf16_to_f64(b)
+    ref = A @ B
+    bound = (K * 2.0 ** -23 + 1e-7) * (np.abs(A) @ np.abs(B)) + 1e-30
+    err = np.abs(c - ref)
+    assert np.all(err <= bound), f"max err {err.max()} at {np.unravel_index(np.argmax(err / bound), err.shape)}"
+
+
+def _map_and_rays(n, seed=0, size=256):
+    img = W.rect_disk_map(size, seed=seed)
+    g = W.grid_poses(img, n, seed=seed + 1)
+    h = g[:, 2]
+    rays = np.stack([g[:, 0], g[:, 1], np.sin(h), np.cos(h)], axis=1).astype(np.float32)
+    return img, g, rays
+
+
+def _targets_for_both_regimes(p, x, R, seed):
+    """Targets around the network's current prediction so the batch exercises
+    both Huber regimes and both sides of the pinball loss."""
+    _, o = S.forward(p, x, emulate_bf16=True)
+    rng = np.random.default_rng(seed)
+    return (R * o[:, 0] + rng.choice([-9.0, -0.4, 0.3, 2.0, 15.0], o.shape[0])).astype(np.float32)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _cos(a, b):
+    return float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("n", [3000, 64, 777])
+def test_features_loss_gradient_vs_cpu(ctx, n):
+    R = 120.0
+    img, g, rays = _map_and_rays(n)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=3)
+    p, freq = sg.params()
+    p = p.copy()
+    p[S.B4 + 1] = 1.0  # bound e^1 cells: both sides of the pinball loss occur
+    sg.set_params(p)
+    # features: bf16 of the float64 features, up to fp32 phase rounding
+    _, _, feat = sg.gradient(R, rays, np.zeros(n, np.float32))
+    x = S.bf16_to_f64(feat)
+    want = S.features(rays, freq, img.shape[1], img.shape[0])
+    assert np.max(np.abs(x - want)) <= 2.0 ** -8 + 1e-4
+    t = _targets_for_both_regimes(p, x, R, n)
+    loss, grad, feat2 = sg.gradient(R, rays, t)
+    assert np.array_equal(feat, feat2)
+    l16, g16, _ = S.loss_and_grad(p, x, t, R, emulate_bf16=True)
+    l64, g64, _ = S.loss_and_grad(p, x, t, R)
+    assert abs(loss - l16) <= 1e-4 * abs(l16)
+    report = []
+    for k, (w, b, o, i) in enumerate(S.LAYERS):
+        for name, sl in ((f"W{k + 1}", slice(w, w + o * i)), (f"b{k + 1}", slice(b, b + o))):
+            r16, c16 = _rel(grad[sl], g16[sl]), _cos(grad[sl], g16[sl])
+            r64, c64 = _rel(grad[sl], g64[sl]), _cos(grad[sl], g64[sl])
+            report.append(f"{name}: rel {r16:.2e} cos {c16:.6f} | fp64 rel {r64:.2e} cos {c64:.5f}")
+            assert r16 <= 1e-2 and c16 >= 0.9999, report[-1]
+            assert c64 >= 0.99, report[-1]
+    print("\n".join(report))
+
+
+def test_adam_steps_vs_cpu(ctx):
+    R = 120.0
+    n = 2048
+    img, g, rays = _map_and_rays(n, seed=5)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=4)
+    p0, freq = sg.params()
+    want = O.Oracle(img, R, "rm").calc_range_many(g).astype(np.float32)
+    p = p0.astype(np.float64)
+    m1 = np.zeros_like(p)
+    m2 = np.zeros_like(p)
+    for t in (1, 2, 3):
+        _, grad, feat = sg.gradient(R, rays, want)
+        sg.step(1e-3)
+        got, _ = sg.params()
+        p, m1, m2 = S.adam_step(p, m1, m2, grad.astype(np.float64), t, 1e-3)
+        assert np.allclose(got, p, rtol=1e-5, atol=2e-7), f"step {t}: max {np.max(np.abs(got - p))}"
+        p = got.astype(np.float64)  # follow the device's fp32 parameters
+    # the forward pass uses the updated bf16 weights
+    out = sg.forward(rays)
+    _, o = S.forward(p, S.bf16_to_f64(feat), emulate_bf16=True)
+    assert np.max(np.abs(out - o)) <= 1e-3 * max(1.0, np.max(np.abs(o)))
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    img = W.rect_disk_map(256, seed=7)
+    R = 120.0
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=1)
+    curve = sg.train(R, n_pairs=16384 * 256, batch=16384, lr=1e-3, seed=2)
+    return img, dm, sg, R, curve
+
+
+def test_training_reduces_loss(trained):
+    img, dm, sg, R, curve = trained
+    assert curve.shape == (256,) and np.all(np.isfinite(curve))
+    first, last = float(np.mean(curve[:4])), float(np.mean(curve[-16:]))
+    print(f"loss {first:.3f} -> {last:.3f}")
+    assert last < 0.5 * first
+
+
+def _exact_and_checked(trained, bound_max, n_part=500, n_beams=60, seed=9):
+    torch = _torch()
+    img, dm, sg, R, _ = trained
+    ins = W.world_poses(img, n_part, seed=seed)
+    ang = W.beam_angles(n_beams)
+    d_ins = torch.from_numpy(ins).cuda()
+    d_ang = torch.from_numpy(ang).cuda()
+    exact = torch.empty(n_part * n_beams, dtype=torch.float32, device="cuda")
+    got = torch.full((n_part * n_beams,), -1.0, dtype=torch.float32, device="cuda")
+    dm.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, R, d_ins.data_ptr(), d_ang.data_ptr(), exact.data_ptr(),
+                                   n_part, n_beams)
+    st = sg.calc_range_angles_dev(R, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(), n_part, n_beams,
+                                  bound_max=bound_max)
+    cnt = torch.empty(n_part * n_beams, dtype=torch.int32, device="cuda")
+    dm.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, R, d_ins.data_ptr(), d_ang.data_ptr(),
+                       cnt.data_ptr(), n_part, n_beams)
+    torch.cuda.synchronize()
+    return exact.cpu().numpy(), got.cpu().numpy(), st, int(cnt.sum().item())
+
+
+def test_checked_inference_fallback_is_exact(trained):
+    exact, got, st, reads = _exact_and_checked(trained, bound_max=-1.0)
+    n = exact.size
+    assert st["fallback_bound"] == n and st["fallback_check"] == 0 and st["from_start"] == 0
+    assert_bits_equal(got, exact, "all rays marched from the origin")
+    assert st["reads"] == reads
+
+
+def test_checked_inference_error_and_reads(trained):
+    exact, got, st, reads = _exact_and_checked(trained, bound_max=0.5)
+    n = exact.size
+    assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == n
+    err = np.abs(got.astype(np.float64) - exact)
+    q = np.quantile(err, [0.5, 0.9, 0.99])
+    print(f"from start {st['from_start'] / n:.3f}, fallback bound {st['fallback_bound'] / n:.3f}, "
+          f"check {st['fallback_check'] / n:.3f}; err p50 {q[0]:.3g} p90 {q[1]:.3g} p99 {q[2]:.3g} "
+          f"max {err.max():.3g}; reads/ray {st['reads'] / n:.2f} vs exact {reads / n:.2f}")
+    assert np.all(np.isfinite(got)) and np.all(got >= 0.0) and np.all(got <= 120.0)
+    assert st["reads"] <= reads  # the start point only removes reads
+
+
+def test_checked_inference_untrained_any_bound(ctx):
+    """An untrained network with an unlimited bound: every ray tries the start
+    point; results stay within [0, max_range] and the counters add up."""
+    torch = _torch()
+    img = W.corridor_map(300, seed=3)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=11)
+    ins = torch.from_numpy(W.world_poses(img, 37, seed=1)).cuda()
+    ang = torch.from_numpy(W.beam_angles(13)).cuda()
+    out = torch.full((37 * 13,), -1.0, dtype=torch.float32, device="cuda")
+    st = sg.calc_range_angles_dev(80.0, ins.data_ptr(), ang.data_ptr(), out.data_ptr(), 37, 13, bound_max=1e30)
+    o = out.cpu().numpy()
+    assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == 37 * 13
+    assert np.all(np.isfinite(o)) and np.all((o >= 0) & (o <= 80.0))
+
+
+def test_surrogate_rejects_bad_arguments(ctx):
+    img = W.rect_disk_map(64, seed=1)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=0)
+    with pytest.raises(rl.RangeLibError):
+        sg.train(10.0, n_pairs=100, batch=16)  # batch below 64
+    with pytest.raises(ValueError):
+        sg.set_params(np.zeros(5, np.float32))
