@@ -156,6 +156,83 @@ def bench_uneven64(args):
         "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}))
 
 
+def bench_surrogate(args):
+    """Secondary workload: the learned surrogate on the particle-filter step's
+    map (2048x2048 corridors, max_range 500).  Trains on ``--sg-pairs`` rays
+    marched on the GPU (uniform free-space origins and headings, minibatch
+    65536), then casts the step's 4000 x 60 rays through the checked start
+    point at several bound limits and compares them with the exact marcher:
+    error distribution, share of rays finished from the start point, map reads
+    per ray, and time per step against the plain marching kernel."""
+    import torch
+
+    import range_libc_amd as rl
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    img, ins, angles, table, true_pose, seed_noise = build_workload()
+    m = rl.DeviceMap(ctx, np.ascontiguousarray(img.T.astype(np.uint8)))
+    sg = rl.Surrogate(m, seed=0)
+    batch = 65536
+    t0 = time.perf_counter()
+    curve = sg.train(MAX_RANGE, n_pairs=args.sg_pairs, batch=batch, lr=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    n = PARTICLES * BEAMS
+    d_ins = torch.from_numpy(ins).to(dev)
+    d_ang = torch.from_numpy(angles).to(dev)
+    exact = torch.empty(n, dtype=torch.float32, device=dev)
+    got = torch.empty(n, dtype=torch.float32, device=dev)
+    cnt = torch.empty(n, dtype=torch.int32, device=dev)
+    m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(),
+                                  exact.data_ptr(), PARTICLES, BEAMS)
+    m.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(),
+                      cnt.data_ptr(), PARTICLES, BEAMS)
+    stream.synchronize()
+    ex = exact.cpu().numpy().astype(np.float64)
+    exact_reads = float(cnt.sum().item()) / n
+    rows = []
+    for bmax in (0.5, 2.0, 4.0, 8.0, 16.0, 1e30):
+        st = sg.calc_range_angles_dev(MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(), PARTICLES,
+                                      BEAMS, bound_max=bmax)
+        err = np.abs(got.cpu().numpy().astype(np.float64) - ex)
+        q = np.quantile(err, [0.5, 0.9, 0.99])
+        rows.append({"bound_max": bmax, "from_start": st["from_start"] / n,
+                     "fallback_bound": st["fallback_bound"] / n, "fallback_check": st["fallback_check"] / n,
+                     "reads_per_ray": st["reads"] / n, "err_p50": q[0], "err_p90": q[1], "err_p99": q[2],
+                     "err_max": float(err.max()), "exact_share": float(np.mean(err == 0.0))})
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda._sleep(50_000_000)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    plain_ms = timed(lambda: m.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, MAX_RANGE, d_ins.data_ptr(),
+                                                           d_ang.data_ptr(), exact.data_ptr(), PARTICLES, BEAMS),
+                     args.steps)
+    sg_ms = timed(lambda: sg.calc_range_angles_dev(MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(),
+                                                   PARTICLES, BEAMS, bound_max=8.0, stats=False), args.steps)
+    flops = 2.0 * n * (64 * 256 + 2 * 256 * 256 + 256 * 2)
+    print(json.dumps({
+        "metric": "surrogate rays/s relative to the plain GPU marcher (particle-filter step rays)",
+        "value": plain_ms / sg_ms, "unit": "x", "surrogate_ms_per_step": sg_ms, "plain_ms_per_step": plain_ms,
+        "surrogate_rays_per_s": n / (sg_ms * 1e-3), "plain_rays_per_s": n / (plain_ms * 1e-3),
+        "network_tflops_per_step": flops / 1e12, "network_tflops_s_over_whole_step": flops / (sg_ms * 1e-3) / 1e12,
+        "train_pairs": int(curve.size * batch), "train_s": train_s,
+        "train_pairs_per_s": curve.size * batch / train_s,
+        "loss_first": float(np.mean(curve[:4])), "loss_last": float(np.mean(curve[-16:])),
+        "exact_reads_per_ray": exact_reads, "by_bound": rows, "timed_bound_max": 8.0,
+        "config": {"workload": "surrogate_corridor2048_p4000_b60_r500", "batch": batch}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,8 +242,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
-    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64"],
-                    help="pf_step (the headline particle-filter step) or uneven64 (multi-map batch)")
+    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate"],
+                    help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
+                         "surrogate (learned start point on the step's map)")
+    ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -181,6 +260,8 @@ def main():
     dev = torch.device("cuda", 0)
     if args.workload == "uneven64":
         return bench_uneven64(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)

@@ -64,7 +64,10 @@ int rl_abi_version(void);
 /* Binds a HIP device, creates a stream, uploads the sinf/cosf exception table.
  * Replaces the device set-up of RayMarchingCUDA::RayMarchingCUDA (kernels.cu:205-218). */
 int rl_create(int device, rl_ctx** out);
-/* Replaces RayMarchingCUDA::~RayMarchingCUDA (kernels.cu:220-222). */
+/* Replaces RayMarchingCUDA::~RayMarchingCUDA (kernels.cu:220-222).  Handles
+ * are reference counted: maps and batches keep their context alive, and
+ * surrogates their map, so destroying them in any order is safe; the caller's
+ * handle is invalid after its destroy call either way. */
 int rl_destroy(rl_ctx* ctx);
 /* Orders all later work of the context on `hip_stream` (a hipStream_t; NULL
  * restores the context's own stream).  No counterpart in the reference. */
@@ -194,7 +197,7 @@ int rl_batch_launch(rl_batch* batch);
  * bf16 weights with fp32 accumulation on the matrix cores, trained on the GPU on
  * rays cast by the exact marcher.  Inference is a checked start point for ray
  * marching: a ray whose predicted bound is <= bound_max cells starts at
- * max(0, range - 2) if that cell is free and must hit by range + 2 + bound;
+ * max(0, range - bound - 1) if that cell is free and must hit by range + bound + 2;
  * every other ray is marched from its origin.  Results approximate the exact
  * march (a thin wall before the start point is missed); the error is reported,
  * not bounded.  A surrogate belongs to one map and must not outlive it. */

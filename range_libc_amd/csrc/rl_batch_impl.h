@@ -196,6 +196,7 @@ int rl_batch_create(rl_ctx* c, int method, rl_batch** out) {
   TRY(check_method(method));
   rl_batch* b = new rl_batch;
   b->ctx = c;
+  c->refs.fetch_add(1);
   b->method = method;
   *out = b;
   return RL_OK;
@@ -207,8 +208,9 @@ int rl_batch_destroy(rl_batch* b) {
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->d_maps) (void)hipFree(b->d_maps);
   if (b->d_items) (void)hipFree(b->d_items);
+  rl_ctx* c = b->ctx;
   delete b;
-  return RL_OK;
+  return rl_destroy(c);  // drops the batch's reference
 }
 
 int rl_batch_set_jobs(rl_batch* b, const rl_batch_job* jobs, int n_jobs) {

@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -609,6 +610,9 @@ struct rl_ctx {
   float* d_lut = nullptr;  // RL_LUT_SIZE tabulated distances (see rl_device.h)
   int xcd_sched = 0;          // RL_OPT_XCD_SCHED
   int lds_box = 0;            // RL_OPT_LDS_BOX
+  // the caller's handle plus one per map / batch created on this context: the
+  // context is freed when the last of them is destroyed, in any order
+  std::atomic<int> refs{1};
   std::mutex mu;
   void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
@@ -634,6 +638,7 @@ struct rl_ctx {
 
 struct rl_map {
   rl_ctx* ctx = nullptr;
+  std::atomic<int> refs{1};  // the caller's handle plus one per surrogate of this map
   int width = 0, height = 0;
   float* d_edt = nullptr;
   uint8_t* d_occ = nullptr;
@@ -951,6 +956,7 @@ int rl_create(int device, rl_ctx** out) {
 
 int rl_destroy(rl_ctx* c) {
   if (!c) return RL_OK;
+  if (c->refs.fetch_sub(1) != 1) return RL_OK;  // maps or batches still hold it
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 6; ++i)
@@ -987,6 +993,7 @@ int rl_map_create_ex(rl_ctx* c, const uint8_t* occ, int width, int height, unsig
   TRY(use_device(c));
   rl_map* m = new rl_map;
   m->ctx = c;
+  c->refs.fetch_add(1);
   m->width = width;
   m->height = height;
   set_world(m, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f);  // RangeLibc.pyx:174-180
@@ -1016,17 +1023,17 @@ int rl_map_create(rl_ctx* c, const uint8_t* occ, int width, int height, rl_map**
 
 int rl_map_destroy(rl_map* m) {
   if (!m) return RL_OK;
-  if (m->ctx) {
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
-  }
+  if (m->refs.fetch_sub(1) != 1) return RL_OK;  // a surrogate still holds it
+  rl_ctx* c = m->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   if (m->d_occ) (void)hipFree(m->d_occ);
   if (m->d_edt) (void)hipFree(m->d_edt);
   if (m->d_idx8) (void)hipFree(m->d_idx8);
   if (m->d_sched) (void)hipFree(m->d_sched);
   if (m->d_table) (void)hipFree(m->d_table);
   delete m;
-  return RL_OK;
+  return rl_destroy(c);  // drops the map's reference
 }
 
 int rl_map_set_world(rl_map* m, float scale, float angle, float ox, float oy, float sin_a,

@@ -16,11 +16,13 @@
 // pinball loss at quantile 0.95 between the bound and the observed absolute
 // error (the error is treated as a constant for this term).
 //
-// Checked inference: a ray whose bound is <= 0.5 cells starts marching at
-// t0 = max(0, range - 2) instead of 0 if the cell at t0 is free and inside the
-// map; the march from t0 must hit within range + 2 + bound, otherwise (and
-// when the bound is larger) the ray is marched from 0.  The result is an
-// approximation whose error against the exact march is measured and reported.
+// Checked inference: a ray whose bound is <= bound_max cells starts marching
+// at t0 = max(0, range - bound - 1) instead of 0 if the cell at t0 is free and
+// inside the map; the march from t0 must hit by range + bound + 2, otherwise
+// (and when the bound is larger) the ray is marched from 0.  A wall before t0
+// is not seen, and the samples from t0 need not land in the exact march's hit
+// cell, so the result approximates the exact march; its error is measured,
+// not bounded.
 
 #define RL_SG_F 64    // Fourier features
 #define RL_SG_NF 32   // frequencies
@@ -329,9 +331,9 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
     bool tried = false;
     if (bound <= bound_max && pred >= 0.0f && pred <= max_range && rl::fast_march_ok(x0, y0, max_range)) {
       tried = true;
-      float t = pred - 2.0f;
+      float t = pred - bound - 1.0f;
       if (t < 0.0f) t = 0.0f;
-      const float t_end = pred + 2.0f + bound;
+      const float t_end = pred + bound + 2.0f;
       int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
       if ((unsigned)px < W && (unsigned)py < H && m.edt[(size_t)px * H + py] > 0.0f) {
         while (t < max_range && t <= t_end) {
@@ -601,6 +603,7 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
   TRY(use_device(map->ctx));
   rl_surrogate* s = new rl_surrogate;
   s->map = map;
+  map->refs.fetch_add(1);
   int rc = RL_OK;
   if ((rc = sg_alloc(&s->d_p, RL_SG_NPARAM)) || (rc = sg_alloc(&s->d_m1, RL_SG_NPARAM)) ||
       (rc = sg_alloc(&s->d_m2, RL_SG_NPARAM)) || (rc = sg_alloc(&s->d_g, RL_SG_NPARAM)) ||
@@ -608,11 +611,13 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
       (rc = sg_alloc(&s->d_stats, 4))) {
     sg_free(s);
     delete s;
+    map->refs.fetch_sub(1);  // cannot reach zero: the caller holds the map
     return rc;
   }
   // He-normal weights, zero biases; the range head starts at a tenth of
-  // max_range, the bound head at e^2 cells.  Frequencies: N(0, 12^2) for the
-  // normalised position, N(0, 2^2) for the direction components.
+  // max_range, the bound head at e^2 cells.  Frequencies: N(0, 2^2) for the
+  // normalised position, N(0, 1) for the direction components (low
+  // frequencies train fastest on both small rooms and large corridor maps).
   std::mt19937_64 rng(seed);
   std::vector<float> p(RL_SG_NPARAM, 0.0f), fr(RL_SG_NF * 4);
   auto fill = [&](int off, int fout, int fin) {
@@ -629,7 +634,7 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
   p[SgLayout::B4] = 0.1f;
   p[SgLayout::B4 + 1] = 2.0f;
   {
-    std::normal_distribution<double> pos(0.0, 12.0), dir(0.0, 2.0);
+    std::normal_distribution<double> pos(0.0, 2.0), dir(0.0, 1.0);
     for (int j = 0; j < RL_SG_NF; ++j) {
       fr[4 * j] = (float)pos(rng);
       fr[4 * j + 1] = (float)pos(rng);
@@ -651,6 +656,7 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
   if (e != hipSuccess) {
     sg_free(s);
     delete s;
+    map->refs.fetch_sub(1);
     return fail(RL_ERR_HIP, "surrogate init: %s", hipGetErrorString(e));
   }
   *out = s;
@@ -659,11 +665,12 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
 
 int rl_surrogate_destroy(rl_surrogate* s) {
   if (!s) return RL_OK;
-  (void)hipSetDevice(s->map->ctx->device);
-  (void)hipStreamSynchronize(s->map->ctx->stream);
+  rl_map* m = s->map;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
   sg_free(s);
   delete s;
-  return RL_OK;
+  return rl_map_destroy(m);  // drops the surrogate's reference
 }
 
 int rl_surrogate_num_params(void) { return RL_SG_NPARAM; }

@@ -166,8 +166,13 @@ def test_checked_inference_fallback_is_exact(trained):
     assert st["reads"] == reads
 
 
-def test_checked_inference_error_and_reads(trained):
-    exact, got, st, reads = _exact_and_checked(trained, bound_max=0.5)
+@pytest.mark.parametrize("bound_max", [0.5, 8.0])
+def test_checked_inference_error_and_reads(trained, bound_max):
+    """What the start point gains (distance-map reads per ray) and costs
+    (error against the exact march), printed.  A ray whose check fails pays
+    for both marches, so fewer reads in total is a property of a trained
+    network, asserted when a fifth of the rays finish from the start point."""
+    exact, got, st, reads = _exact_and_checked(trained, bound_max=bound_max)
     n = exact.size
     assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == n
     err = np.abs(got.astype(np.float64) - exact)
@@ -175,8 +180,10 @@ def test_checked_inference_error_and_reads(trained):
     print(f"from start {st['from_start'] / n:.3f}, fallback bound {st['fallback_bound'] / n:.3f}, "
           f"check {st['fallback_check'] / n:.3f}; err p50 {q[0]:.3g} p90 {q[1]:.3g} p99 {q[2]:.3g} "
           f"max {err.max():.3g}; reads/ray {st['reads'] / n:.2f} vs exact {reads / n:.2f}")
-    assert np.all(np.isfinite(got)) and np.all(got >= 0.0) and np.all(got <= 120.0)
-    assert st["reads"] <= reads  # the start point only removes reads
+    # a hit cell's centre distance may exceed max_range by up to sqrt(2) cells
+    assert np.all(np.isfinite(got)) and np.all(got >= 0.0) and np.all(got <= 120.0 + 1.5)
+    if st["from_start"] > n // 5:
+        assert st["reads"] < reads
 
 
 def test_checked_inference_untrained_any_bound(ctx):
@@ -192,7 +199,7 @@ def test_checked_inference_untrained_any_bound(ctx):
     st = sg.calc_range_angles_dev(80.0, ins.data_ptr(), ang.data_ptr(), out.data_ptr(), 37, 13, bound_max=1e30)
     o = out.cpu().numpy()
     assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == 37 * 13
-    assert np.all(np.isfinite(o)) and np.all((o >= 0) & (o <= 80.0))
+    assert np.all(np.isfinite(o)) and np.all((o >= 0) & (o <= 80.0 + 1.5))
 
 
 def test_surrogate_rejects_bad_arguments(ctx):
