@@ -13,7 +13,23 @@
 
 #pragma clang fp contract(off)
 
+// Explicit address spaces for the hot loads: a load through a generic pointer
+// is a flat access, which the compiler can only wait for together with every
+// other outstanding load (vmcnt(0) lgkmcnt(0)); global and LDS pointers give
+// global_load / ds_read with their own counters.
+#define RL_GLB __attribute__((address_space(1)))
+#define RL_LDS __attribute__((address_space(3)))
+
 namespace rl {
+
+template <class T>
+__device__ __forceinline__ const RL_GLB T* glb(const T* p) {
+  return (const RL_GLB T*)p;
+}
+template <class T>
+__device__ __forceinline__ const RL_LDS T* lds(const T* p) {
+  return (const RL_LDS T*)p;
+}
 
 struct TrigDev {
   rltrig::Bitmap sin_exc, cos_exc;  // exception bitmaps of the C library's sinf / cosf
@@ -69,7 +85,7 @@ __device__ __forceinline__ float rm_march(const float* __restrict__ edt, int W, 
       if (COUNT) *reads = nr;
       return max_range;
     }
-    float d = edt[(size_t)px * (size_t)H + (size_t)py];
+    float d = glb(edt)[(size_t)px * (size_t)H + (size_t)py];
     if (COUNT) ++nr;
     if (d <= 0.0f) {
       float xd = (float)px - x0;
@@ -124,47 +140,130 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
   return b;
 }
 
+#ifndef RL_SPEC
+#define RL_SPEC 1  // positions loaded per round of the speculative march (1: plain)
+#endif
+#ifndef RL_SPEC_AFTER
+#define RL_SPEC_AFTER 16  // reads of a ray before it starts to speculate
+#endif
+
+// Code of grid cell (ux, uy) of the compact map: from the LDS box when BOX and
+// the box covers the cell, else from the map in HBM.
+template <bool BOX>
+__device__ __forceinline__ unsigned idx8_code(const MapDev& m, const Box& box, unsigned ux, unsigned uy) {
+  const unsigned inner = ((ux & 7u) << 3) | (uy & 7u);
+  if (BOX) {
+    unsigned ib = (unsigned)((int)(ux >> 3) - box.bx0), jb = (unsigned)((int)(uy >> 3) - box.by0);
+    if (ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS) return lds(box.lds)[((ib * RL_BOX_BLOCKS + jb) << 6) | inner];
+  }
+  return glb(m.idx8)[((size_t)((ux >> 3) * (unsigned)m.nby + (uy >> 3)) << 6) | inner];
+}
+
+// Distance of a cell with non-zero code c.
+__device__ __forceinline__ float idx8_dist(const MapDev& m, const float* lut, unsigned c, int px, int py) {
+  return (c != RL_IDX8_ESCAPE) ? lds(lut)[c] : glb(m.edt)[(size_t)px * (unsigned)m.height + py];
+}
+
 // The same march over the compact map.  `lut` is the distance table staged in
 // LDS.  Valid when |x0|, |y0| and max_range are below 1e9 (checked by the
 // caller): every x0 + dx*t then lies in int range, so the conversion is a
 // plain truncation.  Identical results to rm_march: a cell's code is 0 exactly
 // when its distance is 0, and any other code maps back to the exact distance.
-// Cells inside the particle box are read from LDS, others from the map.
-template <bool COUNT>
-__device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut, const Box& box,
-                                               float max_range, float x0, float y0, float dx,
-                                               float dy, int* reads) {
-  const unsigned W = (unsigned)m.width, H = (unsigned)m.height, nby = (unsigned)m.nby;
+//
+// Optional speculation (RL_SPEC > 1), after a ray's first RL_SPEC_AFTER reads:
+// each round loads the cells at t, t + g, (t + g) + g, ... (RL_SPEC
+// positions, all loads issued before any is used), where g is the last step
+// taken, and then walks them in order: position k + 1 is the true next
+// position exactly when cell k's step equals g, i.e. when its code equals the
+// code that gave g (distinct codes have distinct steps; escaped cells never
+// match).  The walk stops at the first mismatch and the next round starts from
+// the true next t.  Positions, float operations and results are those of the
+// plain loop, and `reads` counts resolved positions only.
+template <bool COUNT, bool BOX>
+__device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* lut, const Box& box,
+                                                 float max_range, float x0, float y0, float dx,
+                                                 float dy, int* reads) {
+  const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
   float t = 0.0f;
+  float g = 1.0f;       // last step (code 1 is distance 1: step 1)
+  unsigned gcode = 1u;  // the code that gave step g; 256 matches no cell
   int nr = 0;
-  while (t < max_range) {
+  for (int i = 0; (RL_SPEC == 1 || i < RL_SPEC_AFTER) && t < max_range; ++i) {
     int px = (int)(x0 + dx * t);
     int py = (int)(y0 + dy * t);
     if ((unsigned)px >= W || (unsigned)py >= H) {
       if (COUNT) *reads = nr;
       return max_range;
     }
-    unsigned ux = (unsigned)px, uy = (unsigned)py;
-    unsigned inner = ((ux & 7u) << 3) | (uy & 7u);
-    unsigned ib = (unsigned)((int)(ux >> 3) - box.bx0), jb = (unsigned)((int)(uy >> 3) - box.by0);
-    unsigned code;
-    if (box.lds && ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS)
-      code = box.lds[((ib * RL_BOX_BLOCKS + jb) << 6) | inner];
-    else
-      code = m.idx8[((size_t)((ux >> 3) * nby + (uy >> 3)) << 6) | inner];
+    const unsigned c = idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py);
     if (COUNT) ++nr;
-    if (code == 0u) {
+    if (c == 0u) {
       float xd = (float)px - x0;
       float yd = (float)py - y0;
       if (COUNT) *reads = nr;
       return __builtin_sqrtf(xd * xd + yd * yd);
     }
-    float d = (code != RL_IDX8_ESCAPE) ? lut[code] : m.edt[(size_t)ux * H + uy];
-    float st = d * 0.999f;
-    t += (st < 1.0f) ? 1.0f : st;
+    float st = idx8_dist(m, lut, c, px, py) * 0.999f;
+    g = (st < 1.0f) ? 1.0f : st;
+    gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
+    t += g;
+  }
+  if (RL_SPEC > 1) {
+    while (t < max_range) {
+      float tk[RL_SPEC];
+      int px[RL_SPEC], py[RL_SPEC];
+      unsigned code[RL_SPEC];
+      bool inb[RL_SPEC];
+#pragma unroll
+      for (int k = 0; k < RL_SPEC; ++k) {
+        tk[k] = (k == 0) ? t : tk[k - 1] + g;
+        px[k] = (int)(x0 + dx * tk[k]);
+        py[k] = (int)(y0 + dy * tk[k]);
+        inb[k] = (unsigned)px[k] < W && (unsigned)py[k] < H;
+      }
+#pragma unroll
+      for (int k = 0; k < RL_SPEC; ++k)
+        code[k] = idx8_code<BOX>(m, box, inb[k] ? (unsigned)px[k] : 0u, inb[k] ? (unsigned)py[k] : 0u);
+#pragma unroll
+      for (int k = 0; k < RL_SPEC; ++k) {
+        // position k is the true position tk[k], and tk[k] < max_range
+        if (!inb[k]) {
+          if (COUNT) *reads = nr;
+          return max_range;
+        }
+        if (COUNT) ++nr;
+        const unsigned c = code[k];
+        if (c == 0u) {
+          float xd = (float)px[k] - x0;
+          float yd = (float)py[k] - y0;
+          if (COUNT) *reads = nr;
+          return __builtin_sqrtf(xd * xd + yd * yd);
+        }
+        if (c != gcode || k == RL_SPEC - 1) {
+          float st = idx8_dist(m, lut, c, px[k], py[k]) * 0.999f;
+          g = (st < 1.0f) ? 1.0f : st;
+          gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
+          t = tk[k] + g;
+          break;
+        }
+        // same step as the last: the next true position is tk[k + 1]
+        if (!(tk[k + 1 < RL_SPEC ? k + 1 : k] < max_range)) {
+          if (COUNT) *reads = nr;
+          return max_range;
+        }
+      }
+    }
   }
   if (COUNT) *reads = nr;
   return max_range;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut, const Box& box,
+                                               float max_range, float x0, float y0, float dx,
+                                               float dy, int* reads) {
+  if (box.lds) return rm_march_idx8_t<COUNT, true>(m, lut, box, max_range, x0, y0, dx, dy, reads);
+  return rm_march_idx8_t<COUNT, false>(m, lut, box, max_range, x0, y0, dx, dy, reads);
 }
 
 __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_range) {
@@ -175,7 +274,7 @@ __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_rang
 __device__ __forceinline__ bool occupied(const uint8_t* __restrict__ occ, int W, int H, int x,
                                          int y) {
   if (x < 0 || (unsigned)x >= (unsigned)W || y < 0 || (unsigned)y >= (unsigned)H) return false;
-  return occ[(size_t)x * (size_t)H + (size_t)y] != 0;
+  return glb(occ)[(size_t)x * (size_t)H + (size_t)y] != 0;
 }
 
 // BresenhamsLine::calc_range (RangeLib.h:696-769).  (s, c) = (sinf h, cosf h).
@@ -236,7 +335,7 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
     if (0.0f <= _y && _y < limy && 0.0f <= _x && _x < limx) {
       int cx = steep ? (int)_x : (int)_y;
       int cy = steep ? (int)_y : (int)_x;
-      if (occ[(size_t)cx * (size_t)H + (size_t)cy]) {
+      if (glb(occ)[(size_t)cx * (size_t)H + (size_t)cy]) {
         float xd = _x - x0;
         float yd = _y - y0;
         if (COUNT) *visits = nv;

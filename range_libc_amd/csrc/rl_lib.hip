@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
     if (valid && j < M) {
       float r = rl::clamp_table(obs[j] * inv_scale, hi);
       float d = rl::clamp_table(ranges[(size_t)p * M + j] * inv_scale, hi);
-      term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
+      term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
     w = segment_product(w, term, base, min(S, M - c0), S);
   }
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
                                         theta - angles[j], &unused);
       d = rl::clamp_table(d, hi);
       float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
-      term = table[(size_t)(int)r * (size_t)K + (size_t)(int)d];
+      term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
     w = segment_product(w, term, base, min(S, M - c0), S);
   }
@@ -394,7 +394,7 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
       if (SENSOR) {
         float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
         float dd = rl::clamp_table(d * m.world.inv_scale, hi);
-        term = table[(size_t)(int)r * (size_t)K + (size_t)(int)dd];
+        term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)dd];
       }
     }
     if (SENSOR) w = segment_product(w, term, base, min(S, M - c0), S);

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
       if (t < 0.0f) t = 0.0f;
       const float t_end = pred + bound + 2.0f;
       int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
-      if ((unsigned)px < W && (unsigned)py < H && m.edt[(size_t)px * H + py] > 0.0f) {
+      if ((unsigned)px < W && (unsigned)py < H && rl::glb(m.edt)[(size_t)px * H + py] > 0.0f) {
         while (t < max_range && t <= t_end) {
           px = (int)(x0 + dx * t);
           py = (int)(y0 + dy * t);
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
             result = max_range;
             break;
           }
-          float d = m.edt[(size_t)px * H + py];
+          float d = rl::glb(m.edt)[(size_t)px * H + py];
           ++reads;
           if (d <= 0.0f) {
             float xd = (float)px - x0, yd = (float)py - y0;
