@@ -43,6 +43,7 @@ extern "C" {
  * BresenhamsLine (RangeLib.h:691-772). */
 #define RL_METHOD_RM 0
 #define RL_METHOD_BL 1
+#define RL_METHOD_GLT 2  /* GiantLUTCast: needs rl_map_build_giant_lut first */
 
 /* Coordinate frames of a batch. */
 #define RL_FRAME_GRID 0    /* no conversion: RayMarchingGPU::calc_range_many (RangeLib.h:819-831) */
@@ -90,7 +91,23 @@ int rl_set_option(rl_ctx* ctx, int option, long long value);
 int rl_map_create(rl_ctx* ctx, const uint8_t* occ_xmajor, int width, int height, rl_map** out);
 int rl_map_create_ex(rl_ctx* ctx, const uint8_t* occ_xmajor, int width, int height,
                      unsigned flags, rl_map** out);
+/* A map whose distance transform the caller already has (x-major fp32, the
+ * layout of DistanceTransform::grid flattened as in kernels.cu:213-216): no
+ * transform is computed.  Ray marching then reads these values exactly as
+ * given (the compact map keeps the exact ones, anything else is read from
+ * them); Bresenham uses the occupancy. */
+int rl_map_create_from_edt(rl_ctx* ctx, const uint8_t* occ_xmajor, const float* edt_xmajor, int width,
+                           int height, unsigned flags, rl_map** out);
 int rl_map_destroy(rl_map* map);
+/* GiantLUTCast(OMap, max_range, theta_discretization) (RangeLib.h:1781-1816):
+ * builds the map's W x H x td uint16 table on the device (the reference builds
+ * it on the CPU, one marched ray per entry).  Afterwards RL_METHOD_GLT works in
+ * every casting call at this max_range (RangeLib.h:1866-1877 lookups). */
+int rl_map_build_giant_lut(rl_map* map, float max_range, int theta_discretization);
+/* Copies the table (x-major, theta fastest: the reference's giant_lut[x][y][i])
+ * when `out` is non-NULL; any other pointer may be NULL. */
+int rl_map_get_giant_lut(rl_map* map, uint16_t* out, int* theta_discretization, float* max_range,
+                         float* build_ms);
 /* OMap world_* fields (RangeLib.h:134-141) as RayMarchingCUDA::set_conversion_params
  * consumes them (CudaRangeLib.h:24-35). */
 int rl_map_set_world(rl_map* map, float scale, float angle, float origin_x, float origin_y,

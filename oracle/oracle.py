@@ -46,7 +46,8 @@ class _World(C.Structure):
 class _Method(C.Structure):
     _fields_ = [("kind", C.c_int), ("width", C.c_int), ("height", C.c_int), ("occ", _u8p),
                 ("edt", _fp), ("max_range", C.c_float), ("world", _World), ("table", _dp),
-                ("table_width", C.c_int)]
+                ("table_width", C.c_int), ("glt", C.POINTER(C.c_uint16)), ("glt_td", C.c_int),
+                ("glt_max_div_limits", C.c_float), ("glt_td_div_2pi", C.c_float)]
 
 
 IDENTITY_WORLD = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0)
@@ -69,6 +70,11 @@ def _load_oracle():
     lib.or_numpy_calc_range_angles.argtypes = [mp, _fp, _fp, _fp, C.c_int, C.c_int, _ip]
     lib.or_eval_sensor_model.argtypes = [mp, _fp, _fp, _dp, C.c_int, C.c_int]
     lib.or_calc_range_repeat_angles_eval_sensor_model.argtypes = [mp, _fp, _fp, _fp, _dp, C.c_int, C.c_int]
+    lib.or_glt_build.argtypes = [_fp, C.c_int, C.c_int, C.c_float, C.c_int, C.POINTER(C.c_uint16), _fp]
+    lib.or_glt_discretize.argtypes = [C.c_float, C.c_int, C.c_float]
+    lib.or_glt_discretize.restype = C.c_int
+    lib.or_glt_calc_range.argtypes = [mp, C.c_float, C.c_float, C.c_float]
+    lib.or_glt_calc_range.restype = C.c_float
     lib.or_sinf_many.argtypes = [_fp, _fp, C.c_long]
     lib.or_cosf_many.argtypes = [_fp, _fp, C.c_long]
     return lib
@@ -107,18 +113,41 @@ def libc_cosf(x):
     return out
 
 
-class Oracle:
-    """One range method (``kind`` 'rm' or 'bl') bound to one map, as
-    ``RayMarching(OMap, max_range)`` / ``BresenhamsLine(OMap, max_range)``."""
+def glt_build(edt_xmajor: np.ndarray, max_range: float, theta_disc: int):
+    """GiantLUTCast table (W, H, td) uint16 and (max_div_limits, limits_div_max, td_div_2pi)."""
+    e = np.ascontiguousarray(edt_xmajor, dtype=np.float32)
+    w, h = e.shape
+    table = np.empty((w, h, theta_disc), dtype=np.uint16)
+    consts = np.empty(3, dtype=np.float32)
+    oracle_lib().or_glt_build(_ptr(e, _fp), w, h, max_range, theta_disc,
+                              table.ctypes.data_as(C.POINTER(C.c_uint16)), _ptr(consts, _fp))
+    return table, consts
 
-    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=IDENTITY_WORLD):
+
+def glt_discretize(theta: float, theta_disc: int, td_div_2pi: float) -> int:
+    return oracle_lib().or_glt_discretize(theta, theta_disc, td_div_2pi)
+
+
+class Oracle:
+    """One range method (``kind`` 'rm', 'bl' or 'glt') bound to one map, as
+    ``RayMarching(OMap, max_range)`` / ``BresenhamsLine(OMap, max_range)`` /
+    ``GiantLUTCast(OMap, max_range, theta_disc)``."""
+
+    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=IDENTITY_WORLD,
+                 theta_disc: int = 0):
         self.lib = oracle_lib()
         self.occ = xmajor(img)
         self.width, self.height = self.occ.shape
-        self.edt = edt(img) if kind == "rm" else np.zeros((1, 1), dtype=np.float32)
+        self.edt = edt(img) if kind in ("rm", "glt") else np.zeros((1, 1), dtype=np.float32)
         self.table = np.zeros((1, 1), dtype=np.float64)
         self.m = _Method()
-        self.m.kind = 1 if kind == "bl" else 0
+        self.m.kind = {"rm": 0, "bl": 1, "glt": 2}[kind]
+        if kind == "glt":
+            self.glt, consts = glt_build(self.edt, max_range, theta_disc)
+            self.m.glt = self.glt.ctypes.data_as(C.POINTER(C.c_uint16))
+            self.m.glt_td = theta_disc
+            self.m.glt_max_div_limits = float(consts[0])
+            self.m.glt_td_div_2pi = float(consts[2])
         self.m.width = self.width
         self.m.height = self.height
         self.m.occ = _ptr(self.occ, _u8p)
@@ -200,6 +229,9 @@ def ref_lib():
         lib.ref_edt.argtypes = [_u8p, C.c_int, C.c_int, _fp]
         lib.ref_create.argtypes = [C.c_int, _u8p, C.c_int, C.c_int, C.c_float, _fp]
         lib.ref_create.restype = C.c_void_p
+        lib.ref_create_glt.argtypes = [_u8p, C.c_int, C.c_int, C.c_float, C.c_int, _fp]
+        lib.ref_create_glt.restype = C.c_void_p
+        lib.ref_glt_lut.argtypes = [C.c_void_p, C.POINTER(C.c_uint16), _fp]
         lib.ref_destroy.argtypes = [C.c_void_p]
         lib.ref_set_sensor_model.argtypes = [C.c_void_p, _dp, C.c_int]
         lib.ref_calc_range.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
@@ -237,13 +269,24 @@ def ref_edt(img: np.ndarray) -> np.ndarray:
 class Reference:
     """The reference's RayMarching / BresenhamsLine on one map."""
 
-    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=None):
+    def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=None, theta_disc: int = 0):
         self.lib = ref_lib()
         occ = xmajor(img)
         w, h = occ.shape
+        self.shape = (w, h, theta_disc)
         self._world = None if world is None else np.asarray(world, dtype=np.float32)
         wp = None if self._world is None else _ptr(self._world, _fp)
-        self.h = self.lib.ref_create(1 if kind == "bl" else 0, _ptr(occ, _u8p), w, h, max_range, wp)
+        if kind == "glt":
+            self.h = self.lib.ref_create_glt(_ptr(occ, _u8p), w, h, max_range, theta_disc, wp)
+        else:
+            self.h = self.lib.ref_create(1 if kind == "bl" else 0, _ptr(occ, _u8p), w, h, max_range, wp)
+
+    def glt_table(self):
+        """The reference's GiantLUTCast table (W, H, td) uint16 and (max_div_limits, limits_div_max)."""
+        out = np.empty(self.shape, dtype=np.uint16)
+        consts = np.empty(2, dtype=np.float32)
+        self.lib.ref_glt_lut(self.h, out.ctypes.data_as(C.POINTER(C.c_uint16)), _ptr(consts, _fp))
+        return out, consts
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -35,6 +35,15 @@ struct Handle {
   ranges::RangeMethod* method;
 };
 
+// Read access to GiantLUTCast's protected table and constants: pointers to
+// members formed inside a derived class (never instantiated).
+struct GltPeek : ranges::GiantLUTCast {
+  typedef std::vector<std::vector<std::vector<uint16_t> > > table_t;
+  static table_t ranges::GiantLUTCast::*lut() { return &GltPeek::giant_lut; }
+  static float ranges::GiantLUTCast::*max_div_limits_() { return &GltPeek::max_div_limits; }
+  static float ranges::GiantLUTCast::*limits_div_max_() { return &GltPeek::limits_div_max; }
+};
+
 }  // namespace
 
 extern "C" {
@@ -70,6 +79,28 @@ void* ref_create(int kind, const uint8_t* occ, int w, int h, float max_range, co
   else
     hd->method = new ranges::RayMarching(m, max_range);
   return hd;
+}
+
+// GiantLUTCast(OMap, max_range, theta_discretization) (RangeLib.h:1781-1823):
+// the LUT is built here, on the CPU, by the reference's own constructor.
+void* ref_create_glt(const uint8_t* occ, int w, int h, float max_range, int theta_disc, const float* world) {
+  ranges::OMap m = make_map(occ, w, h, world);
+  Handle* hd = new Handle;
+  hd->kind = 2;
+  hd->method = new ranges::GiantLUTCast(m, max_range, theta_disc);
+  return hd;
+}
+
+// The LUT as built (giant_lut[x][y][i], x-major, theta fastest) and its
+// scale constants (max_div_limits, limits_div_max).
+void ref_glt_lut(void* p, uint16_t* out, float* consts) {
+  ranges::GiantLUTCast* g = (ranges::GiantLUTCast*)((Handle*)p)->method;
+  size_t k = 0;
+  for (auto& sx : g->*GltPeek::lut())
+    for (auto& sy : sx)
+      for (uint16_t v : sy) out[k++] = v;
+  consts[0] = g->*GltPeek::max_div_limits_();
+  consts[1] = g->*GltPeek::limits_div_max_();
 }
 
 void ref_destroy(void* p) {

@@ -176,14 +176,75 @@ float or_bl_calc_range(const uint8_t* occ, int width, int height, float max_rang
   return max_range;
 }
 
+/* ---------------------------------------------------------------------------
+ * GiantLUTCast (RangeLib.h:1772-1900) with _GIANT_LUT_SHORT_DATATYPE,
+ * _USE_CACHED_CONSTANTS and _USE_ALTERNATE_MOD set, _USE_FAST_ROUND clear
+ * (RangeLib.h:66-75).  M_2PI is the double literal of RangeLib.h:61. */
+#define OR_M_2PI 6.28318530718
+
+void or_glt_build(const float* edt, int width, int height, float max_range, int td, uint16_t* table,
+                  float* consts) {
+  /* RangeLib.h:1783-1786: int / double, double / float, float / int, int / float */
+  const float td_div_2pi = (float)((double)td / OR_M_2PI);
+  const float step = (float)(OR_M_2PI / (double)(float)td);
+  const float max_div_limits = max_range / (float)65535;
+  const float limits_div_max = (float)65535 / max_range;
+  for (int x = 0; x < width; ++x)
+    for (int y = 0; y < height; ++y)
+      for (int i = 0; i < td; ++i) {
+        const float angle = (float)i * step;                                   /* :1796 */
+        float r = or_rm_calc_range(edt, width, height, max_range, (float)x, (float)y, angle, NULL);
+        r = (r < max_range) ? r : max_range;                                   /* std::min, :1803 */
+        table[((size_t)x * height + y) * td + i] = (uint16_t)(r * limits_div_max); /* :1805 */
+      }
+  if (consts) {
+    consts[0] = max_div_limits;
+    consts[1] = limits_div_max;
+    consts[2] = td_div_2pi;
+  }
+}
+
+int or_glt_discretize(float theta, int td, float td_div_2pi) {
+  if (theta != theta) return -1;
+  if (theta < 0.0) {                                                             /* :1834-1837 */
+    while (theta < 0.0) {
+      float next = (float)((double)theta + OR_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  } else if (theta > OR_M_2PI) {                                                 /* :1838-1841 */
+    while (theta > OR_M_2PI) {
+      float next = (float)((double)theta - OR_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  }
+  int rounded = (int)roundf(theta * td_div_2pi);                                /* :1854 */
+  return rounded % td;                                                           /* :1858 */
+}
+
+float or_glt_calc_range(const or_method* m, float x, float y, float heading) {
+  /* RangeLib.h:1866-1870 */
+  if (x < 0 || x >= (float)m->width || y < 0 || y >= (float)m->height) return m->max_range;
+  if (x != x || y != y) return NAN; /* the reference indexes its table at (int)NaN */
+  int b = or_glt_discretize(heading, m->glt_td, m->glt_td_div_2pi);
+  if (b < 0) return NAN;
+  return (float)m->glt[((size_t)(int)x * m->height + (int)y) * m->glt_td + b] * m->glt_max_div_limits;
+}
+
 float or_calc_range(const or_method* m, float x, float y, float heading) {
   if (m->kind == 1)
     return or_bl_calc_range(m->occ, m->width, m->height, m->max_range, x, y, heading, NULL);
+  if (m->kind == 2) return or_glt_calc_range(m, x, y, heading);
   return or_rm_calc_range(m->edt, m->width, m->height, m->max_range, x, y, heading, NULL);
 }
 
 static float calc_range_counted(const or_method* m, float x, float y, float heading, int* cnt) {
   if (m->kind == 1) return or_bl_calc_range(m->occ, m->width, m->height, m->max_range, x, y, heading, cnt);
+  if (m->kind == 2) {
+    if (cnt) *cnt = (x < 0 || x >= (float)m->width || y < 0 || y >= (float)m->height) ? 0 : 1;
+    return or_glt_calc_range(m, x, y, heading);
+  }
   return or_rm_calc_range(m->edt, m->width, m->height, m->max_range, x, y, heading, cnt);
 }
 

@@ -20,6 +20,10 @@
  *   or_numpy_*          golden vectors (identity and non-identity world params);
  *                       identity-parameter closed form of the transform.
  *   or_eval_sensor_*    golden vectors; constant-table closed form c^M.
+ *   or_glt_*            the compiled reference's own GiantLUTCast (live, and a
+ *                       golden fixture of its table and queries); the table
+ *                       entry equals the quantized or_rm_calc_range, and bins
+ *                       of headings i * 2pi/td are i.
  */
 #ifndef RL_ORACLE_H
 #define RL_ORACLE_H
@@ -37,7 +41,7 @@ typedef struct {
 
 /* One range method bound to a map (RangeMethod state, RangeLib.h:682-688). */
 typedef struct {
-  int kind;            /* 0 = RayMarching, 1 = BresenhamsLine */
+  int kind;            /* 0 = RayMarching, 1 = BresenhamsLine, 2 = GiantLUTCast */
   int width, height;   /* grid[x][y], x < width, y < height */
   const uint8_t* occ;  /* occupancy, x-major: occ[x*height + y] */
   const float* edt;    /* distance transform, x-major (RayMarching only) */
@@ -45,6 +49,9 @@ typedef struct {
   or_world world;
   const double* table; /* sensor model, row-major K x K (row = observed) */
   int table_width;
+  const uint16_t* glt; /* GiantLUTCast table, glt[(x*height + y)*glt_td + bin] */
+  int glt_td;          /* theta discretization */
+  float glt_max_div_limits, glt_td_div_2pi;
 } or_method;
 
 /* DistanceTransform(OMap*) (RangeLib.h:345-373). */
@@ -57,6 +64,22 @@ float or_rm_calc_range(const float* edt, int width, int height, float max_range,
 /* BresenhamsLine::calc_range (RangeLib.h:696-769); *visits = cells tested. */
 float or_bl_calc_range(const uint8_t* occ, int width, int height, float max_range, float x,
                        float y, float heading, int* visits);
+
+/* GiantLUTCast construction (RangeLib.h:1781-1816): table[(x*height + y)*td + i]
+ * = (uint16) (min(R, RM(x, y, i * (2pi/td))) * (65535 / R)); consts receives
+ * max_div_limits, limits_div_max, theta_discretization_div_M_2PI. */
+void or_glt_build(const float* edt, int width, int height, float max_range, int td, uint16_t* table,
+                  float* consts);
+
+/* GiantLUTCast::discretize_theta (RangeLib.h:1832-1864, _USE_ALTERNATE_MOD,
+ * _USE_CACHED_CONSTANTS, roundf).  Returns -1 where the reference does not
+ * terminate (a heading so large that subtracting 2pi leaves it unchanged) or
+ * reads outside its table (NaN heading). */
+int or_glt_discretize(float theta, int td, float td_div_2pi);
+
+/* GiantLUTCast::calc_range (RangeLib.h:1866-1877); NaN where or_glt_discretize is -1
+ * and for a NaN origin (the reference indexes its table at (int)NaN). */
+float or_glt_calc_range(const or_method* m, float x, float y, float heading);
 
 float or_calc_range(const or_method* m, float x, float y, float heading);
 

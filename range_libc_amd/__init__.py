@@ -18,11 +18,11 @@ import sys
 import numpy as np
 
 from . import _lib
-from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL, RL_METHOD_RM, Batch,
+from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL, RL_METHOD_GLT, RL_METHOD_RM, Batch,
                    Context, DeviceMap, RangeLibError, Surrogate, default_context)
 
 __all__ = [
-    "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyNull",
+    "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyGiantLUTCast", "PyNull",
     "RangeLibError", "Context", "DeviceMap", "Batch", "Surrogate", "default_context",
     "USE_CACHED_TRIG", "USE_ALTERNATE_MOD", "USE_CACHED_CONSTANTS", "USE_FAST_ROUND", "NO_INLINE",
     "USE_LRU_CACHE", "LRU_CACHE_SIZE", "SHOULD_USE_CUDA",
@@ -245,6 +245,25 @@ class PyBresenhamsLine(_GpuMethod):
 
     def calc_range(self, x, y, heading):
         return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))
+
+
+class PyGiantLUTCast(_GpuMethod):
+    """GiantLUTCast (RangeLib.h:1772-1900; RangeLibc.pyx:289-312): the table is built on
+    the GPU at construction, one marched ray per (x, y, theta bin)."""
+
+    METHOD = RL_METHOD_GLT
+
+    def __init__(self, Map: PyOMap, max_range: float, theta_disc: int, device: int = 0):
+        super().__init__(Map, max_range, device)
+        self.theta_disc = int(theta_disc)
+        self.map.build_giant_lut(self.max_range, self.theta_disc)
+
+    def calc_range(self, x, y, heading):
+        return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))
+
+    def lut_size(self):
+        """Bytes of the table (RangeLib.h:1818-1820)."""
+        return self.map.width * self.map.height * self.theta_disc * 2
 
 
 class PyNull:

@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("RL_LIBRARY", os.path.join(PKG, "librangelib_hip.so"))
 
 RL_METHOD_RM = 0
 RL_METHOD_BL = 1
+RL_METHOD_GLT = 2
 RL_FRAME_GRID = 0
 RL_FRAME_WORLD = 1
 RL_FRAME_ANGLES = 2
@@ -39,7 +40,10 @@ SIGNATURES = {
     "rl_set_option": [_vp, C.c_int, C.c_longlong],
     "rl_map_create": [_vp, _u8p, C.c_int, C.c_int, C.POINTER(_vp)],
     "rl_map_create_ex": [_vp, _u8p, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)],
+    "rl_map_create_from_edt": [_vp, _u8p, _fp, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)],
     "rl_map_destroy": [_vp],
+    "rl_map_build_giant_lut": [_vp, C.c_float, C.c_int],
+    "rl_map_get_giant_lut": [_vp, _vp, C.POINTER(C.c_int), _fp, _fp],
     "rl_map_set_world": [_vp] + [C.c_float] * 6,
     "rl_map_set_sensor_table": [_vp, _dp, C.c_int],
     "rl_map_get_edt": [_vp, _fp],
@@ -191,15 +195,22 @@ class DeviceMap:
     ``occ_xmajor`` is uint8 (width, height): ``occ[x, y]`` = grid[x][y].
     """
 
-    def __init__(self, ctx: Context, occ_xmajor: np.ndarray, flags: int = 0):
+    def __init__(self, ctx: Context, occ_xmajor: np.ndarray, flags: int = 0, edt_xmajor: np.ndarray | None = None):
         occ = np.ascontiguousarray(occ_xmajor, dtype=np.uint8)
         if occ.ndim != 2:
             raise ValueError("occupancy must be 2-D (width, height)")
         self.ctx = ctx
         self.width, self.height = int(occ.shape[0]), int(occ.shape[1])
         h = C.c_void_p()
-        check(lib().rl_map_create_ex(ctx.handle, occ.ctypes.data_as(_u8p), self.width, self.height,
-                                     flags, C.byref(h)))
+        if edt_xmajor is None:
+            check(lib().rl_map_create_ex(ctx.handle, occ.ctypes.data_as(_u8p), self.width, self.height,
+                                         flags, C.byref(h)))
+        else:
+            edt = np.ascontiguousarray(edt_xmajor, dtype=np.float32)
+            if edt.shape != occ.shape:
+                raise ValueError("distance transform and occupancy shapes differ")
+            check(lib().rl_map_create_from_edt(ctx.handle, occ.ctypes.data_as(_u8p), edt.ctypes.data_as(_fp),
+                                               self.width, self.height, flags, C.byref(h)))
         self.handle = h
         self.has_table = False
 
@@ -222,6 +233,20 @@ class DeviceMap:
         t = np.ascontiguousarray(table, dtype=np.float64)
         check(lib().rl_map_set_sensor_table(self.handle, t.ctypes.data_as(_dp), t.shape[0]))
         self.has_table = True
+
+    def build_giant_lut(self, max_range: float, theta_disc: int):
+        """GiantLUTCast table of this map (RL_METHOD_GLT then works at this max_range)."""
+        check(lib().rl_map_build_giant_lut(self.handle, max_range, int(theta_disc)))
+
+    def giant_lut(self, copy: bool = True):
+        """(table (W, H, td) uint16 or None, td, max_range, build_ms)."""
+        td, r, ms = C.c_int(), C.c_float(), C.c_float()
+        check(lib().rl_map_get_giant_lut(self.handle, None, C.byref(td), C.byref(r), C.byref(ms)))
+        out = None
+        if copy:
+            out = np.empty((self.width, self.height, td.value), dtype=np.uint16)
+            check(lib().rl_map_get_giant_lut(self.handle, C.c_void_p(out.ctypes.data), None, None, None))
+        return out, td.value, r.value, ms.value
 
     def edt(self) -> np.ndarray:
         out = np.empty((self.width, self.height), dtype=np.float32)

@@ -160,6 +160,9 @@ int batch_launch_locked(rl_batch* b) {
   if (b->method == RL_METHOD_BL)
     hipLaunchKernelGGL(k_batch<RL_METHOD_BL>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
                        b->n_items, c->trig, b->lds_box);
+  else if (b->method == RL_METHOD_GLT)
+    hipLaunchKernelGGL(k_batch<RL_METHOD_GLT>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
+                       b->n_items, c->trig, b->lds_box);
   else
     hipLaunchKernelGGL(k_batch<RL_METHOD_RM>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
                        b->n_items, c->trig, b->lds_box);
@@ -224,7 +227,10 @@ int rl_batch_set_jobs(rl_batch* b, const rl_batch_job* jobs, int n_jobs) {
     if ((long long)j.num_particles * j.num_angles > 0x7fffffffLL) return fail(RL_ERR_ARG, "job %d: too many rays", i);
     if (j.d_obs && (!j.d_weights || !j.map->d_table))
       return fail(RL_ERR_ARG, "job %d: the sensor model needs weights and a sensor table", i);
-    if (b->method == RL_METHOD_RM && !j.map->d_edt) return fail(RL_ERR_STATE, "job %d: map has no distance transform", i);
+    if (check_map(j.map, b->method, j.max_range) != RL_OK) {
+      std::string msg = g_err;
+      return fail(RL_ERR_STATE, "job %d: %s", i, msg.c_str());
+    }
   }
   std::lock_guard<std::mutex> lk(b->ctx->mu);
   TRY(use_device(b->ctx));

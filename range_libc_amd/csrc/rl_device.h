@@ -58,6 +58,10 @@ struct MapDev {
   int nby;              // blocks along y: (height + 7) / 8
   int use_box;          // stage the 64x64 cells around a particle in LDS
   WorldDev world;
+  // GiantLUTCast table (null when not built): glt[(x*height + y)*glt_td + bin]
+  const uint16_t* glt;
+  int glt_td;
+  float glt_R, glt_scale, glt_td_div_2pi;  // its max_range, max_div_limits, td / M_2PI
 };
 
 __device__ __forceinline__ void sincos_libc(float h, const TrigDev& t, float* s, float* c) {
@@ -269,6 +273,46 @@ __device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut
                                                float dy, int* reads) {
   if (box.lds) return rm_march_idx8_t<COUNT, true>(m, lut, box, max_range, x0, y0, dx, dy, reads);
   return rm_march_idx8_t<COUNT, false>(m, lut, box, max_range, x0, y0, dx, dy, reads);
+}
+
+// GiantLUTCast::discretize_theta (RangeLib.h:1832-1864) with
+// _USE_ALTERNATE_MOD and _USE_CACHED_CONSTANTS (RangeLib.h:66-67): the wrap
+// loops add or subtract the double M_2PI (RangeLib.h:61) and round back to
+// float each time, then roundf(theta * (td / M_2PI)) % td.  -1 for a NaN
+// heading (the reference indexes out of its table) and for a heading so large
+// that a wrap step leaves it unchanged (the reference loops forever); every
+// terminating case (|theta| below about 1.3e8) is followed exactly.
+#define RL_M_2PI 6.28318530718
+__device__ __forceinline__ int glt_bin(float theta, int td, float td_div_2pi) {
+  if (theta != theta) return -1;
+  if (theta < 0.0) {
+    while (theta < 0.0) {
+      const float next = (float)((double)theta + RL_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  } else if (theta > RL_M_2PI) {
+    while (theta > RL_M_2PI) {
+      const float next = (float)((double)theta - RL_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  }
+  return (int)roundf(theta * td_div_2pi) % td;
+}
+
+// GiantLUTCast::calc_range (RangeLib.h:1866-1877).  NaN where glt_bin is -1
+// and for a NaN origin (the reference converts it to an out-of-table index).
+template <bool COUNT>
+__device__ __forceinline__ float glt_calc_range(const MapDev& m, float x, float y, float heading, int* reads) {
+  if (COUNT) *reads = 0;
+  if (x < 0 || x >= (float)m.width || y < 0 || y >= (float)m.height) return m.glt_R;
+  if (x != x || y != y) return __builtin_nanf("");
+  const int b = glt_bin(heading, m.glt_td, m.glt_td_div_2pi);
+  if (b < 0) return __builtin_nanf("");
+  if (COUNT) *reads = 1;
+  const size_t cell = (size_t)(int)x * (unsigned)m.height + (unsigned)(int)y;
+  return (float)glb(m.glt)[cell * (unsigned)m.glt_td + (unsigned)b] * m.glt_scale;
 }
 
 __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_range) {

@@ -180,6 +180,7 @@ template <int METHOD, bool COUNT>
 __device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, const rl::Box& box,
                                           const TrigDev& trig, float max_range, float x, float y,
                                           float heading, int* cnt) {
+  if (METHOD == RL_METHOD_GLT) return rl::glt_calc_range<COUNT>(m, x, y, heading, cnt);
   float s, c;
   rl::sincos_libc(heading, trig, &s, &c);
   if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
@@ -451,6 +452,30 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
                                  p, M, S, j0, base, boxes + wib * RL_BOX_BYTES);
 }
 
+// GiantLUTCast table (RangeLib.h:1781-1816): entry k = (cell * td + i), cell =
+// x * H + y, is the ray marched from the integer point (x, y) at angle
+// i * step, clamped to max_range and quantized to uint16; `step` and
+// `limits_div_max` are the reference's float constants.  One ray per thread
+// (grid-stride); neighbouring threads share a start cell.
+__global__ __launch_bounds__(256) void k_glt_build(MapDev m, TrigDev trig, float max_range, int td, float step,
+                                                   float limits_div_max, uint16_t* __restrict__ out,
+                                                   long long total) {
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<RL_METHOD_RM>(m, lut);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+    const long long cell = k / td;
+    const int i = (int)(k - cell * td);
+    const int x = (int)(cell / m.height), y = (int)(cell - (long long)x * m.height);
+    const float angle = (float)i * step;
+    int unused;
+    float r = cast_one<RL_METHOD_RM, false>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, (float)x, (float)y,
+                                            angle, &unused);
+    r = (r < max_range) ? r : max_range;  // std::min(max_range, r)
+    out[k] = (uint16_t)(unsigned)(r * limits_div_max);
+  }
+}
+
 // Compact index map from the fp32 distance transform (layout in rl_device.h).
 __global__ void k_build_idx8(const float* __restrict__ edt, const float* __restrict__ lut,
                              uint8_t* __restrict__ idx8, int W, int H, int nby, size_t nbytes) {
@@ -463,12 +488,16 @@ __global__ void k_build_idx8(const float* __restrict__ edt, const float* __restr
   unsigned code = RL_IDX8_ESCAPE;
   if (px < (unsigned)W && py < (unsigned)H) {
     float d = edt[(size_t)px * H + py];
-    int lo = 0, hi = RL_IDX8_ESCAPE - 1;  // lut[0..254] strictly increasing
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (lut[mid] < d) lo = mid + 1; else hi = mid;
+    if (d <= 0.0f) {
+      code = 0u;  // the march stops at any d <= 0 and does not use its value
+    } else {
+      int lo = 0, hi = RL_IDX8_ESCAPE - 1;  // lut[0..254] strictly increasing
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (lut[mid] < d) lo = mid + 1; else hi = mid;
+      }
+      if (lut[lo] == d) code = (unsigned)lo;
     }
-    if (lut[lo] == d) code = (unsigned)lo;
   }
   idx8[i] = (uint8_t)code;
 }
@@ -650,9 +679,13 @@ struct rl_map {
   int nby = 0;
   WorldDev world{};
   float edt_ms = 0.0f;
+  uint16_t* d_glt = nullptr;  // GiantLUTCast table (rl_map_build_giant_lut)
+  int glt_td = 0;
+  float glt_R = 0.0f, glt_scale = 0.0f, glt_td_div_2pi = 0.0f, glt_ms = 0.0f;
 
   MapDev dev() const {
-    return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut, width, height, nby, ctx->lds_box, world};
+    return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut, width, height, nby, ctx->lds_box, world,
+                  d_glt, glt_td, glt_R, glt_scale, glt_td_div_2pi};
   }
 };
 
@@ -670,10 +703,15 @@ void set_world(rl_map* m, float scale, float angle, float ox, float oy, float si
 }
 
 int check_method(int method) {
-  if (method != RL_METHOD_RM && method != RL_METHOD_BL)
+  if (method != RL_METHOD_RM && method != RL_METHOD_BL && method != RL_METHOD_GLT)
     return fail(RL_ERR_ARG, "unknown range method %d", method);
   return RL_OK;
 }
+
+// The method can run on this map at this max_range: ray marching needs the
+// distance transform; a GiantLUTCast table answers only for the max_range it
+// was built with (its quantization and out-of-map value depend on it).
+int check_map(const rl_map* m, int method, float max_range);
 
 #define TRY(expr)              \
   do {                         \
@@ -697,6 +735,8 @@ int launch_cast_f(rl_map* m, int method, float max_range, const float* ins, cons
                   float* outs, int* counts, int n_rays, int num_angles) {
   if (method == RL_METHOD_BL)
     return launch_cast_t<RL_METHOD_BL, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  if (method == RL_METHOD_GLT)
+    return launch_cast_t<RL_METHOD_GLT, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
   return launch_cast_t<RL_METHOD_RM, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
 }
 
@@ -705,8 +745,7 @@ int launch_particles(rl_map* m, int method, bool sensor, float max_range, const 
 
 int launch_cast(rl_map* m, int method, int frame, bool count, float max_range, const float* ins,
                 const float* angles, float* outs, int* counts, int n_rays, int num_angles) {
-  TRY(check_method(method));
-  if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  TRY(check_map(m, method, max_range));
   if (frame == RL_FRAME_GRID)
     return count ? launch_cast_f<0, true>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles)
                  : launch_cast_f<0, false>(m, method, max_range, ins, angles, outs, counts, n_rays, num_angles);
@@ -735,7 +774,7 @@ int launch_sensor(rl_map* m, const float* obs, const float* ranges, double* outs
 
 int launch_fused(rl_map* m, int method, float max_range, const float* ins, const float* angles,
                  const float* obs, double* weights, int N, int M) {
-  TRY(check_method(method));
+  TRY(check_map(m, method, max_range));
   if (!m->d_table) return fail(RL_ERR_STATE, "no sensor model set (call set_sensor_model first)");
   if (N == 0) return RL_OK;
   int S = segment_size(M);
@@ -743,6 +782,9 @@ int launch_fused(rl_map* m, int method, float max_range, const float* ins, const
   dim3 grid(grid_for(waves * 64, 256)), block(256);
   if (method == RL_METHOD_BL)
     hipLaunchKernelGGL(k_fused<RL_METHOD_BL>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
+                       max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
+  else if (method == RL_METHOD_GLT)
+    hipLaunchKernelGGL(k_fused<RL_METHOD_GLT>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
                        max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
   else
     hipLaunchKernelGGL(k_fused<RL_METHOD_RM>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
@@ -797,18 +839,44 @@ int launch_particles_t(rl_map* m, float max_range, const float* ins, const float
 
 int launch_particles(rl_map* m, int method, bool sensor, float max_range, const float* ins,
                      const float* angles, const float* obs, float* ranges, double* weights, int N, int M) {
-  TRY(check_method(method));
+  TRY(check_map(m, method, max_range));
   if (sensor && !m->d_table) return fail(RL_ERR_STATE, "no sensor model set (call set_sensor_model first)");
-  if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
   if (N == 0) return RL_OK;
   if (method == RL_METHOD_BL)
     return sensor ? launch_particles_t<RL_METHOD_BL, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
                   : launch_particles_t<RL_METHOD_BL, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
+  if (method == RL_METHOD_GLT)
+    return sensor ? launch_particles_t<RL_METHOD_GLT, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
+                  : launch_particles_t<RL_METHOD_GLT, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
   return sensor ? launch_particles_t<RL_METHOD_RM, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
                 : launch_particles_t<RL_METHOD_RM, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
 }
 
 // Builds the distance transform of m->d_occ into m->d_edt.
+// Compact map from m->d_edt (unless RL_MAP_NO_COMPACT).  Every value <= 0
+// gets code 0 (a hit, as in the fp32 march); any other value that is not a
+// tabulated distance (NaN, not sqrtf of a small integer) gets the escape code,
+// so the compact march reads it from the fp32 map and behaves as the fp32
+// march.
+int build_compact(rl_map* m, unsigned flags) {
+  if (flags & RL_MAP_NO_COMPACT) return RL_OK;
+  rl_ctx* c = m->ctx;
+  const int W = m->width, H = m->height;
+  m->nby = (H + 7) / 8;
+  const size_t nbytes = (size_t)((W + 7) / 8) * (size_t)m->nby * 64;
+  if (hipMalloc(&m->d_idx8, nbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    m->d_idx8 = nullptr;
+    return fail(RL_ERR_NOMEM, "device allocation of the %zu-byte compact map failed", nbytes);
+  }
+  hipLaunchKernelGGL(k_build_idx8, dim3(grid_for((long long)nbytes, 256)), dim3(256), 0, c->stream,
+                     (const float*)m->d_edt, (const float*)c->d_lut, m->d_idx8, W, H, m->nby, nbytes);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(RL_ERR_HIP, "compact map build: %s", hipGetErrorString(e));
+  return RL_OK;
+}
+
 int build_edt(rl_map* m, unsigned flags) {
   rl_ctx* c = m->ctx;
   const int W = m->width, H = m->height;
@@ -850,24 +918,7 @@ int build_edt(rl_map* m, unsigned flags) {
       break;
     }
     (void)hipEventElapsedTime(&m->edt_ms, e0, e1);
-    if (!(flags & RL_MAP_NO_COMPACT)) {
-      m->nby = (H + 7) / 8;
-      const size_t nbytes = (size_t)((W + 7) / 8) * (size_t)m->nby * 64;
-      if (hipMalloc(&m->d_idx8, nbytes) != hipSuccess) {
-        (void)hipGetLastError();
-        m->d_idx8 = nullptr;
-        rc = fail(RL_ERR_NOMEM, "device allocation of the %zu-byte compact map failed", nbytes);
-        break;
-      }
-      hipLaunchKernelGGL(k_build_idx8, dim3(grid_for((long long)nbytes, 256)), dim3(256), 0, c->stream,
-                         (const float*)m->d_edt, (const float*)c->d_lut, m->d_idx8, W, H, m->nby, nbytes);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-      if (e != hipSuccess) {
-        rc = fail(RL_ERR_HIP, "compact map build: %s", hipGetErrorString(e));
-        break;
-      }
-    }
+    rc = build_compact(m, flags);
   } while (0);
   if (g) (void)hipFree(g);
   if (v) (void)hipFree(v);
@@ -875,6 +926,18 @@ int build_edt(rl_map* m, unsigned flags) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+int check_map(const rl_map* m, int method, float max_range) {
+  TRY(check_method(method));
+  if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  if (method == RL_METHOD_GLT) {
+    if (!m->d_glt) return fail(RL_ERR_STATE, "no GiantLUTCast table (call rl_map_build_giant_lut first)");
+    if (!(max_range == m->glt_R))
+      return fail(RL_ERR_ARG, "max_range %g differs from the GiantLUTCast table's %g", (double)max_range,
+                  (double)m->glt_R);
+  }
+  return RL_OK;
 }
 
 int use_device(rl_ctx* c) {
@@ -1021,6 +1084,111 @@ int rl_map_create(rl_ctx* c, const uint8_t* occ, int width, int height, rl_map**
   return rl_map_create_ex(c, occ, width, height, 0u, out);
 }
 
+int rl_map_build_giant_lut(rl_map* m, float max_range, int theta_disc) {
+  CHECK_ARG(m, "map is NULL");
+  CHECK_ARG(theta_disc >= 1 && theta_disc <= 65536, "theta_discretization outside 1..65536");
+  CHECK_ARG(max_range > 0.0f && max_range < 1e9f, "max_range outside (0, 1e9)");
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (!m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  const long long total = (long long)m->width * m->height * theta_disc;
+  if (m->d_glt) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(m->d_glt));
+    m->d_glt = nullptr;
+  }
+  if (hipMalloc(&m->d_glt, (size_t)total * sizeof(uint16_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    m->d_glt = nullptr;
+    return fail(RL_ERR_NOMEM, "device allocation of the %lld-entry GiantLUTCast table failed", total);
+  }
+  // RangeLib.h:1783-1786 (int / double, double / float, float / int, int / float)
+  const double two_pi = 6.28318530718;
+  m->glt_td = theta_disc;
+  m->glt_R = max_range;
+  m->glt_td_div_2pi = (float)((double)theta_disc / two_pi);
+  m->glt_scale = max_range / (float)65535;
+  const float step = (float)(two_pi / (double)(float)theta_disc);
+  const float limits_div_max = (float)65535 / max_range;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  long long blocks = (total + 255) / 256;
+  if (blocks > (1LL << 20)) blocks = 1LL << 20;
+  hipError_t e = hipEventRecord(e0, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_glt_build, dim3((unsigned)blocks), dim3(256), 0, c->stream, m->dev(), c->trig, max_range,
+                       theta_disc, step, limits_div_max, m->d_glt, total);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) (void)hipEventElapsedTime(&m->glt_ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) {
+    (void)hipFree(m->d_glt);
+    m->d_glt = nullptr;
+    return fail(RL_ERR_HIP, "GiantLUTCast build: %s", hipGetErrorString(e));
+  }
+  return RL_OK;
+}
+
+int rl_map_get_giant_lut(rl_map* m, uint16_t* out, int* theta_disc, float* max_range, float* build_ms) {
+  CHECK_ARG(m, "map is NULL");
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (!m->d_glt) return fail(RL_ERR_STATE, "no GiantLUTCast table (call rl_map_build_giant_lut first)");
+  if (theta_disc) *theta_disc = m->glt_td;
+  if (max_range) *max_range = m->glt_R;
+  if (build_ms) *build_ms = m->glt_ms;
+  if (out) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(out, m->d_glt, (size_t)m->width * m->height * m->glt_td * sizeof(uint16_t),
+                      hipMemcpyDeviceToHost));
+  }
+  return RL_OK;
+}
+
+int rl_map_create_from_edt(rl_ctx* c, const uint8_t* occ, const float* edt, int width, int height,
+                           unsigned flags, rl_map** out) {
+  CHECK_ARG(c && occ && edt && out, "null argument");
+  *out = nullptr;
+  if (width < 1 || height < 1 || width > 65535 || height > 65535)
+    return fail(RL_ERR_ARG, "map size %dx%d outside 1..65535", width, height);
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  rl_map* m = new rl_map;
+  m->ctx = c;
+  c->refs.fetch_add(1);
+  m->width = width;
+  m->height = height;
+  set_world(m, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f);
+  const size_t N = (size_t)width * height;
+  if (hipMalloc(&m->d_occ, N) != hipSuccess || hipMalloc(&m->d_edt, N * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    rl_map_destroy(m);
+    return fail(RL_ERR_NOMEM, "device allocation for a %dx%d map failed", width, height);
+  }
+  if (hipMemcpyAsync(m->d_occ, occ, N, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(m->d_edt, edt, N * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    rl_map_destroy(m);
+    return fail(RL_ERR_HIP, "map upload failed");
+  }
+  int rc = build_compact(m, flags);
+  if (rc == RL_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(RL_ERR_HIP, "map upload failed");
+  if (rc != RL_OK) {
+    std::string msg = g_err;
+    rl_map_destroy(m);
+    return fail(rc, "%s", msg.c_str());
+  }
+  *out = m;
+  return RL_OK;
+}
+
 int rl_map_destroy(rl_map* m) {
   if (!m) return RL_OK;
   if (m->refs.fetch_sub(1) != 1) return RL_OK;  // a surrogate still holds it
@@ -1032,6 +1200,7 @@ int rl_map_destroy(rl_map* m) {
   if (m->d_idx8) (void)hipFree(m->d_idx8);
   if (m->d_sched) (void)hipFree(m->d_sched);
   if (m->d_table) (void)hipFree(m->d_table);
+  if (m->d_glt) (void)hipFree(m->d_glt);
   delete m;
   return rl_destroy(c);  // drops the map's reference
 }

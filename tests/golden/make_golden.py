@@ -167,6 +167,52 @@ def main():
                 d[f"{name}_{kind}_{int(mr)}_world"] = ref.numpy_calc_range(eq)
     np.savez_compressed(os.path.join(HERE, "degenerate.npz"), **d)
 
+    giant_lut()
+    listing()
+
+
+def glt_cases():
+    """(name, image, max_range, theta_discretization) of the GiantLUTCast fixtures."""
+    return [("rect64", W.rect_disk_map(64, seed=3), 60.0, 108),
+            ("corr96x80", np.ascontiguousarray(W.corridor_map(96, seed=2)[:, :80]), 120.0, 37),
+            ("rect48", W.rect_disk_map(48, seed=5), 500.0, 360)]
+
+
+def glt_queries(img, td, seed):
+    """Poses over and around the map; headings: bin centres, [-30, 30), [-3000, 3000)."""
+    rng = np.random.default_rng(seed)
+    h, w = img.shape
+    n = 3000
+    heads = np.concatenate([(np.arange(n // 3) % td) * np.float32(2 * math.pi / td), rng.uniform(-30, 30, n // 3),
+                            rng.uniform(-3000, 3000, n - 2 * (n // 3))])
+    return np.stack([rng.uniform(-5, w + 5, n), rng.uniform(-5, h + 5, n), heads], axis=1).astype(np.float32)
+
+
+def giant_lut():
+    """GiantLUTCast (RangeLib.h:1772-1900) built and queried by the compiled reference."""
+    g = {}
+    world = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+    for name, img, R, td in glt_cases():
+        ref = O.Reference(img, R, "glt", theta_disc=td)
+        table, consts = ref.glt_table()
+        g[f"{name}_img"] = img
+        g[f"{name}_table_sha"] = np.array(sha(table))
+        idx = np.random.default_rng(3).integers(0, table.size, 5000).astype(np.int32)
+        g[f"{name}_sample_idx"] = idx
+        g[f"{name}_sample"] = table.reshape(-1)[idx]
+        g[f"{name}_consts"] = consts
+        q = glt_queries(img, td, 4)
+        g[f"{name}_ins"] = q
+        g[f"{name}_grid"] = ref.calc_range_many(q)
+        g[f"{name}_world"] = ref.numpy_calc_range(q)
+        ang = W.beam_angles(61)
+        g[f"{name}_angles"] = ref.numpy_calc_range_angles(q[:300], ang)
+        refw = O.Reference(img, R, "glt", world=world, theta_disc=td)
+        g[f"{name}_world_rot"] = refw.numpy_calc_range(q)
+    np.savez_compressed(os.path.join(HERE, "giant_lut.npz"), **g)
+
+
+def listing():
     for f in sorted(os.listdir(HERE)):
         p = os.path.join(HERE, f)
         if os.path.isfile(p):
@@ -174,4 +220,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["glt"]:
+        giant_lut()
+        listing()
+    else:
+        main()

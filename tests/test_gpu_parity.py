@@ -255,6 +255,43 @@ def test_compact_map_matches_fp32_map_and_oracle(case):
         assert_bits_equal(out.cpu().numpy(), want, f"{case} flags={flags}")
 
 
+def test_map_from_precomputed_edt():
+    """rl_map_create_from_edt: with the exact transform it equals a map that
+    builds its own (and the oracle); with arbitrary values (scaled, negative,
+    -0, NaN) the compact march equals the fp32 march over the same values."""
+    torch = _torch()
+    ctx = rl.default_context(0)
+    img = W.rect_disk_map(300, seed=4)
+    occ = O.xmajor(img)
+    edt = O.edt(img)
+    ins = W.world_poses(img, 700, seed=5)
+    ang = W.beam_angles(60)
+    d_ins, d_ang = torch.from_numpy(ins).cuda(), torch.from_numpy(ang).cuda()
+
+    def cast(dm):
+        out = torch.empty(700 * 60, dtype=torch.float32, device="cuda")
+        dm.numpy_calc_range_angles_dev(rl.RL_METHOD_RM, 200.0, d_ins.data_ptr(), d_ang.data_ptr(),
+                                       out.data_ptr(), 700, 60)
+        ctx.sync()
+        return out.cpu().numpy()
+
+    want = O.Oracle(img, 200.0, "rm").numpy_calc_range_angles(ins, ang)
+    m = rl.DeviceMap(ctx, occ, edt_xmajor=edt)
+    assert np.array_equal(m.edt().view(np.uint32), edt.astype(np.float32).view(np.uint32))
+    assert_bits_equal(cast(m), want, "precomputed exact transform")
+    odd = (edt * np.float32(0.73)).astype(np.float32)
+    rng = np.random.default_rng(6)
+    pick = rng.random(odd.shape)
+    odd[pick < 0.01] = -1.5
+    odd[(pick >= 0.01) & (pick < 0.02)] = -0.0
+    odd[(pick >= 0.02) & (pick < 0.021)] = np.nan
+    a = cast(rl.DeviceMap(ctx, occ, edt_xmajor=odd))
+    b = cast(rl.DeviceMap(ctx, occ, rl._lib.RL_MAP_NO_COMPACT, edt_xmajor=odd))
+    assert_bits_equal(a, b, "compact vs fp32 march over arbitrary values")
+    with pytest.raises(ValueError):
+        rl.DeviceMap(ctx, occ, edt_xmajor=edt[:-1])
+
+
 @pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("bl", 60)])
 def test_pf_step_bit_exact(kind, m, corridor_map):
     """rl_pf_step_dev == numpy_calc_range_angles then eval_sensor_model."""

@@ -314,3 +314,77 @@ def test_oracle_matches_compiled_reference(seed):
         assert np.array_equal(o.eval_sensor_model(obs, rr, 17, 50), r.eval_sensor_model(obs, rr, 17, 50))
         assert np.array_equal(o.calc_range_repeat_angles_eval_sensor_model(q[:50], ang, obs),
                               r.calc_range_repeat_angles_eval_sensor_model(q[:50], ang, obs))
+
+
+# --------------------------------------------------------------------------- GiantLUTCast
+
+def _glt_fixture():
+    return np.load(os.path.join(GOLDEN, "giant_lut.npz"))
+
+
+def _glt_names(g):
+    return sorted({k.rsplit("_img", 1)[0] for k in g.files if k.endswith("_img")})
+
+
+def test_glt_oracle_golden():
+    """Table (hash + samples), constants and queries of the compiled reference's
+    GiantLUTCast equal the oracle's (grid, world, rotated world, repeated angles)."""
+    g = _glt_fixture()
+    world = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+    for name in _glt_names(g):
+        img = g[f"{name}_img"]
+        R = {"rect64": 60.0, "corr96x80": 120.0, "rect48": 500.0}[name]
+        td = {"rect64": 108, "corr96x80": 37, "rect48": 360}[name]
+        o = O.Oracle(img, R, "glt", theta_disc=td)
+        assert hashlib.sha256(np.ascontiguousarray(o.glt).tobytes()).hexdigest() == str(g[f"{name}_table_sha"])
+        assert np.array_equal(o.glt.reshape(-1)[g[f"{name}_sample_idx"]], g[f"{name}_sample"])
+        assert o.m.glt_max_div_limits == float(g[f"{name}_consts"][0])
+        q = g[f"{name}_ins"]
+        assert_bits_equal(o.calc_range_many(q), g[f"{name}_grid"], f"{name} grid")
+        assert_bits_equal(o.numpy_calc_range(q), g[f"{name}_world"], f"{name} world")
+        assert_bits_equal(o.numpy_calc_range_angles(q[:300], W.beam_angles(61)), g[f"{name}_angles"], f"{name} angles")
+        ow = O.Oracle(img, R, "glt", world, theta_disc=td)
+        assert_bits_equal(ow.numpy_calc_range(q), g[f"{name}_world_rot"], f"{name} rotated world")
+
+
+def test_glt_bins_and_entries_closed_form():
+    """A heading i * (2 pi / td) (the build's own angles) falls in bin i; a heading
+    shifted by +-2 pi wraps back to the same bin when the wrap is exact in float;
+    every table entry is the quantized RayMarching range from the cell corner."""
+    td_list = (1, 7, 37, 108, 360, 1000)
+    for td in td_list:
+        step = np.float32(6.28318530718 / np.float64(np.float32(td)))
+        c = np.float32(np.float64(td) / 6.28318530718)
+        for i in range(td):
+            assert O.glt_discretize(float(np.float32(i) * step), td, float(c)) == i
+    assert O.glt_discretize(float("nan"), 108, float(np.float32(108 / 6.28318530718))) == -1
+    assert O.glt_discretize(1e10, 108, float(np.float32(108 / 6.28318530718))) == -1
+    img = W.rect_disk_map(40, seed=9)
+    R, td = 50.0, 24
+    o = O.Oracle(img, R, "glt", theta_disc=td)
+    rm = O.Oracle(img, R, "rm")
+    step = np.float32(6.28318530718 / np.float64(np.float32(td)))
+    ldm = np.float32(65535) / np.float32(R)
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        x, y, i = int(rng.integers(0, 40)), int(rng.integers(0, 40)), int(rng.integers(0, td))
+        r = np.float32(rm.calc_range(float(x), float(y), float(np.float32(i) * step)))
+        r = min(r, np.float32(R))
+        assert o.glt[x, y, i] == np.uint16(np.float32(r * ldm))
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(3))
+def test_glt_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(2000 + seed)
+    h, w = int(rng.integers(2, 50)), int(rng.integers(2, 50))
+    img = (rng.random((h, w)) < rng.uniform(0.01, 0.3)).astype(np.uint8)
+    R, td = float(rng.uniform(5, 200)), int(rng.integers(1, 200))
+    o = O.Oracle(img, R, "glt", theta_disc=td)
+    r = O.Reference(img, R, "glt", theta_disc=td)
+    table, consts = r.glt_table()
+    assert np.array_equal(o.glt, table)
+    q = np.stack([rng.uniform(-3, w + 3, 3000), rng.uniform(-3, h + 3, 3000),
+                  rng.uniform(-100, 100, 3000)], 1).astype(np.float32)
+    assert_bits_equal(o.calc_range_many(q), r.calc_range_many(q), "grid")
+    assert_bits_equal(o.numpy_calc_range(q), r.numpy_calc_range(q), "world")
