@@ -233,6 +233,80 @@ def bench_surrogate(args):
         "config": {"workload": "surrogate_corridor2048_p4000_b60_r500", "batch": batch}}))
 
 
+def bench_glt(args):
+    """Secondary workload: GiantLUTCast on the particle-filter step's map
+    (2048x2048 corridors, max_range 500, 108 heading bins as in the
+    reference's test script): the table built on the GPU (one marched ray per
+    entry, 453 M rays), the reference's CPU constructor timed on a bounded
+    sample (a 256x256 map of the same generator), and the particle-filter step
+    answered from the table."""
+    import torch
+
+    import range_libc_amd as rl
+    from range_libc_amd import workloads as W
+
+    TD = 108
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    img, ins, angles, table, true_pose, seed_noise = build_workload()
+    m = rl.DeviceMap(ctx, np.ascontiguousarray(img.T.astype(np.uint8)))
+    m.set_sensor_table(table)
+    m.build_giant_lut(MAX_RANGE, TD)  # first build: allocation
+    builds = []
+    for _ in range(3):
+        m.build_giant_lut(MAX_RANGE, TD)
+        builds.append(m.giant_lut(copy=False)[3])
+    build_ms = float(np.median(builds))
+    entries = MAP_SIDE * MAP_SIDE * TD
+    d_ins = torch.from_numpy(ins).to(dev)
+    d_ang = torch.from_numpy(angles).to(dev)
+    d_obs = torch.from_numpy(W.noisy_observation(np.full(BEAMS, 50.0, np.float32), seed_noise,
+                                                 max_range=MAX_RANGE)).to(dev)
+    d_rng = torch.empty(PARTICLES * BEAMS, dtype=torch.float32, device=dev)
+    d_w = torch.empty(PARTICLES, dtype=torch.float64, device=dev)
+
+    def step(method):
+        m.pf_step_dev(method, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), d_obs.data_ptr(), d_rng.data_ptr(),
+                      d_w.data_ptr(), PARTICLES, BEAMS)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda._sleep(50_000_000)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    glt_ms = timed(lambda: step(rl.RL_METHOD_GLT), args.steps)
+    rm_ms = timed(lambda: step(rl.RL_METHOD_RM), args.steps)
+    result = {
+        "metric": "GiantLUTCast table entries built per second on 1 GPU (2048^2 corridors, 108 bins, R=500)",
+        "value": entries / (build_ms * 1e-3), "unit": "entries/s", "build_ms": build_ms, "entries": entries,
+        "table_bytes": entries * 2, "pf_step_glt_ms": glt_ms, "pf_step_rm_ms": rm_ms,
+        "pf_step_glt_rays_per_s": PARTICLES * BEAMS / (glt_ms * 1e-3),
+        "config": {"workload": "glt_corridor2048_td108_r500"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        if O.ref_available():
+            small = W.corridor_map(256, 1)
+            t0 = time.perf_counter()
+            O.Reference(small, MAX_RANGE, "glt", theta_disc=TD)
+            dt = time.perf_counter() - t0
+            n_small = 256 * 256 * TD
+            result["cpu_baseline"] = {"value": n_small / dt, "unit": "entries/s", "cores": 1, "kind": "reference",
+                                      "sample": f"GiantLUTCast constructor on a 256x256 corridor map, {TD} bins: "
+                                                f"{n_small} entries in {dt:.2f} s",
+                                      "full_table_estimate_s": entries / (n_small / dt)}
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,7 +316,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
-    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate"],
+    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
                          "surrogate (learned start point on the step's map)")
     ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
@@ -262,6 +336,8 @@ def main():
         return bench_uneven64(args)
     if args.workload == "surrogate":
         return bench_surrogate(args)
+    if args.workload == "glt":
+        return bench_glt(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)
