@@ -261,6 +261,9 @@ int rl_debug_sqrtf_dev(rl_ctx* ctx, const float* d_x, float* d_out, int n);
 /* C = A * B through the surrogate's bf16 matrix-core GEMM (host row-major
  * matrices of bf16 bit patterns; fp32 result). */
 int rl_debug_gemm_bf16(rl_ctx* ctx, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K);
+/* C = A (M x K) * Bt (N x K)^T, the surrogate's forward-layer layout (the
+ * 128 x 128 tile kernel when K % 32 == 0 and N % 128 == 0). */
+int rl_debug_gemm_bf16_nt(rl_ctx* ctx, const uint16_t* A, const uint16_t* Bt, float* C, int M, int N, int K);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,7 @@ SIGNATURES = {
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
     "rl_debug_gemm_bf16": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
+    "rl_debug_gemm_bf16_nt": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "rl_surrogate_create": [_vp, C.c_uint, C.POINTER(_vp)],
     "rl_surrogate_destroy": [_vp],
     "rl_surrogate_num_params": [],
@@ -379,6 +380,19 @@ def debug_gemm_bf16(ctx: Context, a_bits: np.ndarray, b_bits: np.ndarray) -> np.
         raise ValueError("inner dimensions differ")
     c = np.empty((M, N), dtype=np.float32)
     check(lib().rl_debug_gemm_bf16(ctx.handle, ptr(a), ptr(b), ptr(c), M, N, K))
+    return c
+
+
+def debug_gemm_bf16_nt(ctx: Context, a_bits: np.ndarray, bt_bits: np.ndarray) -> np.ndarray:
+    """A (M x K) @ Bt (N x K)^T of bf16 bit patterns; fp32 result (the forward-layer layout)."""
+    a = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    bt = np.ascontiguousarray(bt_bits, dtype=np.uint16)
+    M, K = a.shape
+    N, K2 = bt.shape
+    if K != K2:
+        raise ValueError("inner dimensions differ")
+    c = np.empty((M, N), dtype=np.float32)
+    check(lib().rl_debug_gemm_bf16_nt(ctx.handle, ptr(a), ptr(bt), ptr(c), M, N, K))
     return c
 
 

@@ -152,6 +152,84 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
       }
 }
 
+// Forward-layer GEMM, both operands K-contiguous ("NT"): C(m, n) = sum_k
+// A[m * K + k] * Bt[n * K + k], + bias[n], optional relu, stored fp32 and/or
+// bf16 at C[m * N + n].  Requires K % 32 == 0, N % 128 == 0 and 16-byte
+// aligned rows.  Tile 128 x (32 * WN) per 256-thread workgroup, 64 x (16 * WN)
+// per wave (tiles of v_mfma_f32_16x16x32_bf16), K in steps of 32 through LDS
+// rows padded to 40 elements (16-byte reads land on distinct banks).
+// Operand layout of the 16x16x32 instruction: lane l supplies row (l & 15)
+// of A and column (l & 15) of B, k = 8 * (l >> 4) .. + 7; it receives column
+// (l & 15), rows 4 * (l >> 4) .. + 3 of D.
+typedef __bf16 sg_bf16x8 __attribute__((ext_vector_type(8)));
+#define RL_SG_NT_PAD 40
+
+template <int WN>  // 16-wide MFMA tiles per wave along N: the workgroup tile is 128 x (32 * WN)
+__global__ __launch_bounds__(256) void k_gemm_bf16_nt(const bf16_t* __restrict__ A, const bf16_t* __restrict__ Bt,
+                                                      int M, int N, int K, const float* __restrict__ bias,
+                                                      int relu, float* __restrict__ C, bf16_t* __restrict__ Cb) {
+  constexpr int BN = 32 * WN;
+  __shared__ __align__(16) bf16_t As[128 * RL_SG_NT_PAD];
+  __shared__ __align__(16) bf16_t Bs[BN * RL_SG_NT_PAD];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.y * 128, n0 = blockIdx.x * BN;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * (16 * WN);
+  sg_float4 acc[4][WN];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) acc[i][j] = sg_float4{0.f, 0.f, 0.f, 0.f};
+  const int kq = (lane >> 4) * 8, r16 = lane & 15;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    // 128 rows of A and BN rows of Bt, 4 vectors of 8 elements each
+#pragma unroll
+    for (int v = tid; v < 512; v += 256) {
+      const int r = v >> 2, c = (v & 3) * 8;
+      const int m = m0 + r;
+      uint4 a = make_uint4(0u, 0u, 0u, 0u);
+      if (m < M) a = *(const uint4*)(A + (size_t)m * K + k0 + c);
+      *(uint4*)(As + r * RL_SG_NT_PAD + c) = a;
+    }
+#pragma unroll
+    for (int v = tid; v < BN * 4; v += 256) {
+      const int r = v >> 2, c = (v & 3) * 8;
+      *(uint4*)(Bs + r * RL_SG_NT_PAD + c) = *(const uint4*)(Bt + (size_t)(n0 + r) * K + k0 + c);
+    }
+    __syncthreads();
+    sg_bf16x8 a[4], b[WN];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *(const sg_bf16x8*)(As + (wm + i * 16 + r16) * RL_SG_NT_PAD + kq);
+#pragma unroll
+    for (int j = 0; j < WN; ++j) b[j] = *(const sg_bf16x8*)(Bs + (wn + j * 16 + r16) * RL_SG_NT_PAD + kq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // Fully unrolled so that every accumulator index is a constant (a loop the
+  // compiler keeps would index the accumulators through scratch memory).
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = n0 + wn + j * 16 + r16;
+    const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q;
+        float v = acc[i][j][q] + bn;
+        v = (relu && v < 0.0f) ? 0.0f : v;
+        const size_t o = (size_t)m * N + n;
+        if (m < M) {
+          if (C) C[o] = v;
+          if (Cb) Cb[o] = f2bf(v);
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise kernels
 
@@ -449,9 +527,21 @@ int sg_reserve(rl_surrogate* s, int n) {
   return RL_OK;
 }
 
+// The NT kernel applies when both operands are K-contiguous with 16-byte
+// aligned rows, the output is dense row-major and nothing is masked.
+bool nt_ok(const GemmArgs& g) {
+  return g.sak == 1 && g.sbk == 1 && g.sam == g.K && g.sbn == g.K && g.K % 32 == 0 && g.N % 128 == 0 &&
+         g.scn == 1 && g.scm == g.N && !g.mask && ((uintptr_t)g.A & 15) == 0 && ((uintptr_t)g.B & 15) == 0;
+}
+
 int sg_gemm(hipStream_t st, const GemmArgs& g) {
-  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, g);
+  if (nt_ok(g)) {  // 128 x 128 tiles: 128 x 256 (WN = 8) needs 280 registers, one wave per SIMD, 2x slower
+    dim3 grid(g.N / 128, (g.M + 127) / 128);
+    hipLaunchKernelGGL(k_gemm_bf16_nt<4>, grid, dim3(256), 0, st, g.A, g.B, g.M, g.N, g.K, g.bias, g.relu, g.C, g.Cb);
+  } else {
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, g);
+  }
   HIP_TRY(hipGetLastError());
   return RL_OK;
 }
@@ -591,6 +681,8 @@ int sg_adam(rl_surrogate* s, float lr) {
   HIP_TRY(hipGetLastError());
   return RL_OK;
 }
+
+static int debug_gemm(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, bool b_nk);
 
 }  // namespace
 
@@ -821,9 +913,22 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
   return RL_OK;
 }
 
-// Plain bf16 GEMM through the same kernel (test hook): C = A(M x K) * B(K x N),
+// Plain bf16 GEMM through the same kernels (test hook): C = A(M x K) * B(K x N),
 // row-major host matrices, fp32 result.
 int rl_debug_gemm_bf16(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K) {
+  return debug_gemm(c, A, B, C, M, N, K, false);
+}
+
+// C = A(M x K) * Bt(N x K)^T: the forward-layer layout (K-contiguous operands).
+int rl_debug_gemm_bf16_nt(rl_ctx* c, const uint16_t* A, const uint16_t* Bt, float* C, int M, int N, int K) {
+  return debug_gemm(c, A, Bt, C, M, N, K, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+static int debug_gemm(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, bool b_nk) {
   CHECK_ARG(c && A && B && C && M > 0 && N > 0 && K > 0, "bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
   TRY(use_device(c));
@@ -841,8 +946,8 @@ int rl_debug_gemm_bf16(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C
   g.sam = K;
   g.sak = 1;
   g.B = dB;
-  g.sbk = N;
-  g.sbn = 1;
+  g.sbk = b_nk ? 1 : N;
+  g.sbn = b_nk ? K : 1;
   g.M = M;
   g.N = N;
   g.K = K;
@@ -862,4 +967,4 @@ int rl_debug_gemm_bf16(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C
   return RL_OK;
 }
 
-}  // extern "C"
+}  // namespace

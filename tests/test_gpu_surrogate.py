@@ -40,6 +40,22 @@ def test_gemm_bf16_matches_cpu(ctx, mnk):
     assert np.all(err <= bound), f"max err {err.max()} at {np.unravel_index(np.argmax(err / bound), err.shape)}"
 
 
+@pytest.mark.parametrize("mnk", [(128, 128, 32), (300, 256, 64), (1000, 256, 256), (1, 128, 32), (257, 384, 96),
+                                 (64, 2, 256), (100, 130, 40)])
+def test_gemm_bf16_nt_matches_cpu(ctx, mnk):
+    """The forward-layer layout: the 128 x 128 tile kernel (K % 32 == 0, N % 128 == 0)
+    and the general kernel for the other shapes."""
+    M, N, K = mnk
+    rng = np.random.default_rng(M + 11 * N + 7 * K)
+    a, bt = _rand_bf16(rng, (M, K)), _rand_bf16(rng, (N, K))
+    c = rl._lib.debug_gemm_bf16_nt(ctx, a, bt)
+    A, Bt = S.bf16_to_f64(a), S.bf16_to_f64(bt)
+    ref = A @ Bt.T
+    bound = (K * 2.0 ** -23 + 1e-7) * (np.abs(A) @ np.abs(Bt).T) + 1e-30
+    err = np.abs(c - ref)
+    assert np.all(err <= bound), f"max err {err.max()} at {np.unravel_index(np.argmax(err / bound), err.shape)}"
+
+
 def _map_and_rays(n, seed=0, size=256):
     img = W.rect_disk_map(size, seed=seed)
     g = W.grid_poses(img, n, seed=seed + 1)
