@@ -44,6 +44,7 @@ extern "C" {
 #define RL_METHOD_RM 0
 #define RL_METHOD_BL 1
 #define RL_METHOD_GLT 2  /* GiantLUTCast: needs rl_map_build_giant_lut first */
+#define RL_METHOD_CDDT 3 /* CDDTCast (PCDDT after rl_map_prune_cddt): needs rl_map_build_cddt first */
 
 /* Coordinate frames of a batch. */
 #define RL_FRAME_GRID 0    /* no conversion: RayMarchingGPU::calc_range_many (RangeLib.h:819-831) */
@@ -104,6 +105,16 @@ int rl_map_destroy(rl_map* map);
  * it on the CPU, one marched ray per entry).  Afterwards RL_METHOD_GLT works in
  * every casting call at this max_range (RangeLib.h:1866-1877 lookups). */
 int rl_map_build_giant_lut(rl_map* map, float max_range, int theta_discretization);
+/* CDDTCast(OMap, max_range, theta_discretization) (RangeLib.h:974-1142): the
+ * compressed tables built on the device; RL_METHOD_CDDT then works in every
+ * casting call (RangeLib.h:1341-1514 lookups, any max_range). */
+int rl_map_build_cddt(rl_map* map, int theta_discretization);
+/* CDDTCast::prune(max_range) (RangeLib.h:1172-1275): PCDDT. */
+int rl_map_prune_cddt(rl_map* map, float max_range);
+/* The tables as CSR: widths[td], trans[td] (lut_translations), start[nbins + 1],
+ * vals[nvals]; bins of angle 0 first.  Any pointer may be NULL. */
+int rl_map_get_cddt(rl_map* map, int* theta_discretization, long long* nbins, long long* nvals, int* widths,
+                    float* trans, long long* start, float* vals, float* build_ms);
 /* Copies the table (x-major, theta fastest: the reference's giant_lut[x][y][i])
  * when `out` is non-NULL; any other pointer may be NULL. */
 int rl_map_get_giant_lut(rl_map* map, uint16_t* out, int* theta_discretization, float* max_range,

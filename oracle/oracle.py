@@ -43,11 +43,20 @@ class _World(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("scale", "angle", "origin_x", "origin_y", "sin_angle", "cos_angle")]
 
 
+_llp = C.POINTER(C.c_longlong)
+
+
+class _Cddt(C.Structure):
+    _fields_ = [("td", C.c_int), ("nang", C.c_int), ("width", C.POINTER(C.c_int)), ("trans", _fp), ("cosv", _fp),
+                ("sinv", _fp), ("bin_off", _llp), ("start", _llp), ("vals", _fp), ("td_div_2pi", C.c_float),
+                ("m2pi_div_td", C.c_float)]
+
+
 class _Method(C.Structure):
     _fields_ = [("kind", C.c_int), ("width", C.c_int), ("height", C.c_int), ("occ", _u8p),
                 ("edt", _fp), ("max_range", C.c_float), ("world", _World), ("table", _dp),
                 ("table_width", C.c_int), ("glt", C.POINTER(C.c_uint16)), ("glt_td", C.c_int),
-                ("glt_max_div_limits", C.c_float), ("glt_td_div_2pi", C.c_float)]
+                ("glt_max_div_limits", C.c_float), ("glt_td_div_2pi", C.c_float), ("cddt", C.POINTER(_Cddt))]
 
 
 IDENTITY_WORLD = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0)
@@ -75,6 +84,10 @@ def _load_oracle():
     lib.or_glt_discretize.restype = C.c_int
     lib.or_glt_calc_range.argtypes = [mp, C.c_float, C.c_float, C.c_float]
     lib.or_glt_calc_range.restype = C.c_float
+    lib.or_cddt_build.argtypes = [_u8p, C.c_int, C.c_int, C.c_int]
+    lib.or_cddt_build.restype = C.POINTER(_Cddt)
+    lib.or_cddt_free.argtypes = [C.POINTER(_Cddt)]
+    lib.or_cddt_prune.argtypes = [C.POINTER(_Cddt), _u8p, C.c_int, C.c_int, C.c_float]
     lib.or_sinf_many.argtypes = [_fp, _fp, C.c_long]
     lib.or_cosf_many.argtypes = [_fp, _fp, C.c_long]
     return lib
@@ -128,20 +141,37 @@ def glt_discretize(theta: float, theta_disc: int, td_div_2pi: float) -> int:
     return oracle_lib().or_glt_discretize(theta, theta_disc, td_div_2pi)
 
 
+def cddt_tables(c) -> dict:
+    """CSR view (copies) of an or_cddt: widths, translations, start, vals."""
+    cd = c.contents
+    td = cd.td
+    widths = np.ctypeslib.as_array(cd.width, (td,)).copy()
+    nb = int(widths.sum())
+    start = np.ctypeslib.as_array(cd.start, (nb + 1,)).copy()
+    vals = np.ctypeslib.as_array(cd.vals, (max(int(start[-1]), 1),))[: int(start[-1])].copy()
+    return {"widths": widths, "trans": np.ctypeslib.as_array(cd.trans, (td,)).copy(), "start": start, "vals": vals}
+
+
 class Oracle:
     """One range method (``kind`` 'rm', 'bl' or 'glt') bound to one map, as
     ``RayMarching(OMap, max_range)`` / ``BresenhamsLine(OMap, max_range)`` /
     ``GiantLUTCast(OMap, max_range, theta_disc)``."""
 
     def __init__(self, img: np.ndarray, max_range: float, kind: str = "rm", world=IDENTITY_WORLD,
-                 theta_disc: int = 0):
+                 theta_disc: int = 0, prune: bool = False):
         self.lib = oracle_lib()
         self.occ = xmajor(img)
         self.width, self.height = self.occ.shape
         self.edt = edt(img) if kind in ("rm", "glt") else np.zeros((1, 1), dtype=np.float32)
         self.table = np.zeros((1, 1), dtype=np.float64)
         self.m = _Method()
-        self.m.kind = {"rm": 0, "bl": 1, "glt": 2}[kind]
+        self.m.kind = {"rm": 0, "bl": 1, "glt": 2, "cddt": 3}[kind]
+        self._cddt = None
+        if kind == "cddt":
+            self._cddt = self.lib.or_cddt_build(_ptr(self.occ, _u8p), self.width, self.height, theta_disc)
+            if prune:
+                self.lib.or_cddt_prune(self._cddt, _ptr(self.occ, _u8p), self.width, self.height, max_range)
+            self.m.cddt = self._cddt
         if kind == "glt":
             self.glt, consts = glt_build(self.edt, max_range, theta_disc)
             self.m.glt = self.glt.ctypes.data_as(C.POINTER(C.c_uint16))
@@ -156,6 +186,14 @@ class Oracle:
         self.m.world = _World(*[float(v) for v in world])
         self.m.table = _ptr(self.table, _dp)
         self.m.table_width = 1
+
+    def __del__(self):
+        if getattr(self, "_cddt", None):
+            self.lib.or_cddt_free(self._cddt)
+            self._cddt = None
+
+    def cddt_tables(self) -> dict:
+        return cddt_tables(self._cddt)
 
     def set_sensor_model(self, table: np.ndarray):
         self.table = np.ascontiguousarray(table, dtype=np.float64)
@@ -232,6 +270,11 @@ def ref_lib():
         lib.ref_create_glt.argtypes = [_u8p, C.c_int, C.c_int, C.c_float, C.c_int, _fp]
         lib.ref_create_glt.restype = C.c_void_p
         lib.ref_glt_lut.argtypes = [C.c_void_p, C.POINTER(C.c_uint16), _fp]
+        lib.ref_create_cddt.argtypes = [_u8p, C.c_int, C.c_int, C.c_float, C.c_int, _fp]
+        lib.ref_create_cddt.restype = C.c_void_p
+        lib.ref_cddt_prune.argtypes = [C.c_void_p, C.c_float]
+        lib.ref_cddt_sizes.argtypes = [C.c_void_p, _llp, _llp]
+        lib.ref_cddt_export.argtypes = [C.c_void_p, C.POINTER(C.c_int), _fp, _llp, _fp]
         lib.ref_destroy.argtypes = [C.c_void_p]
         lib.ref_set_sensor_model.argtypes = [C.c_void_p, _dp, C.c_int]
         lib.ref_calc_range.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
@@ -278,8 +321,25 @@ class Reference:
         wp = None if self._world is None else _ptr(self._world, _fp)
         if kind == "glt":
             self.h = self.lib.ref_create_glt(_ptr(occ, _u8p), w, h, max_range, theta_disc, wp)
+        elif kind == "cddt":
+            self.h = self.lib.ref_create_cddt(_ptr(occ, _u8p), w, h, max_range, theta_disc, wp)
         else:
             self.h = self.lib.ref_create(1 if kind == "bl" else 0, _ptr(occ, _u8p), w, h, max_range, wp)
+
+    def cddt_prune(self, max_range: float):
+        self.lib.ref_cddt_prune(self.h, max_range)
+
+    def cddt_tables(self) -> dict:
+        nb, nv = C.c_longlong(), C.c_longlong()
+        self.lib.ref_cddt_sizes(self.h, C.byref(nb), C.byref(nv))
+        td = self.shape[2]
+        widths = np.empty(td, np.int32)
+        trans = np.empty(td, np.float32)
+        start = np.empty(nb.value + 1, np.int64)
+        vals = np.empty(max(nv.value, 1), np.float32)
+        self.lib.ref_cddt_export(self.h, widths.ctypes.data_as(C.POINTER(C.c_int)), _ptr(trans, _fp),
+                                 start.ctypes.data_as(_llp), _ptr(vals, _fp))
+        return {"widths": widths, "trans": trans, "start": start, "vals": vals[: nv.value]}
 
     def glt_table(self):
         """The reference's GiantLUTCast table (W, H, td) uint16 and (max_div_limits, limits_div_max)."""

@@ -103,6 +103,47 @@ void ref_glt_lut(void* p, uint16_t* out, float* consts) {
   consts[1] = g->*GltPeek::limits_div_max_();
 }
 
+// CDDTCast(OMap, max_range, theta_discretization) (RangeLib.h:974-1142).
+void* ref_create_cddt(const uint8_t* occ, int w, int h, float max_range, int theta_disc, const float* world) {
+  ranges::OMap m = make_map(occ, w, h, world);
+  Handle* hd = new Handle;
+  hd->kind = 3;
+  hd->method = new ranges::CDDTCast(m, max_range, theta_disc);
+  return hd;
+}
+
+void ref_cddt_prune(void* p, float max_range) {
+  ((ranges::CDDTCast*)((Handle*)p)->method)->prune(max_range);
+}
+
+// Sizes of compressed_lut: number of bins over all angles, number of values.
+void ref_cddt_sizes(void* p, long long* nbins, long long* nvals) {
+  ranges::CDDTCast* c = (ranges::CDDTCast*)((Handle*)p)->method;
+  long long b = 0, v = 0;
+  for (auto& ang : c->compressed_lut)
+    for (auto& bin : ang) {
+      ++b;
+      v += (long long)bin.size();
+    }
+  *nbins = b;
+  *nvals = v;
+}
+
+// compressed_lut as CSR (bins of angle 0 first), per-angle widths and lut_translations.
+void ref_cddt_export(void* p, int* widths, float* trans, long long* start, float* vals) {
+  ranges::CDDTCast* c = (ranges::CDDTCast*)((Handle*)p)->method;
+  long long b = 0, v = 0;
+  start[0] = 0;
+  for (size_t a = 0; a < c->compressed_lut.size(); ++a) {
+    widths[a] = (int)c->compressed_lut[a].size();
+    trans[a] = c->lut_translations[a];
+    for (auto& bin : c->compressed_lut[a]) {
+      for (float f : bin) vals[v++] = f;
+      start[++b] = v;
+    }
+  }
+}
+
 void ref_destroy(void* p) {
   Handle* hd = (Handle*)p;
   delete hd->method;

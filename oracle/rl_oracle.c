@@ -7,6 +7,7 @@
 #include "rl_oracle.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -236,6 +237,7 @@ float or_calc_range(const or_method* m, float x, float y, float heading) {
   if (m->kind == 1)
     return or_bl_calc_range(m->occ, m->width, m->height, m->max_range, x, y, heading, NULL);
   if (m->kind == 2) return or_glt_calc_range(m, x, y, heading);
+  if (m->kind == 3) return or_cddt_calc_range(m->cddt, m->occ, m->width, m->height, m->max_range, x, y, heading);
   return or_rm_calc_range(m->edt, m->width, m->height, m->max_range, x, y, heading, NULL);
 }
 
@@ -244,6 +246,10 @@ static float calc_range_counted(const or_method* m, float x, float y, float head
   if (m->kind == 2) {
     if (cnt) *cnt = (x < 0 || x >= (float)m->width || y < 0 || y >= (float)m->height) ? 0 : 1;
     return or_glt_calc_range(m, x, y, heading);
+  }
+  if (m->kind == 3) {
+    if (cnt) *cnt = 0;
+    return or_cddt_calc_range(m->cddt, m->occ, m->width, m->height, m->max_range, x, y, heading);
   }
   return or_rm_calc_range(m->edt, m->width, m->height, m->max_range, x, y, heading, cnt);
 }
@@ -359,4 +365,248 @@ void or_sinf_many(const float* x, float* out, long n) {
 }
 void or_cosf_many(const float* x, float* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = cosf(x[i]);
+}
+
+/* ---------------------------------------------------------------------------
+ * CDDTCast (RangeLib.h:971-1770): compressed directional distance transform,
+ * with _USE_CACHED_TRIG clear, _USE_CACHED_CONSTANTS and _USE_ALTERNATE_MOD
+ * set, _EPSILON 1e-5 and _BINARY_SEARCH_THRESHOLD 64 (RangeLib.h:60-67).  The
+ * bins of angle a are compressed_lut[a][i]; stored here as one CSR array. */
+#define OR_EPS 0.00001
+#define OR_BSEARCH 64
+#define OR_M_PI 3.14159265358979323846
+
+static int cmp_float(const void* a, const void* b) {
+  float x = *(const float*)a, y = *(const float*)b;
+  return (x > y) - (x < y);
+}
+
+static int edge_cell(const uint8_t* occ, int w, int h, int x, int y) {
+  /* OMap::make_edge_map(true) (RangeLib.h:293-310), utils::outline (RangeUtils.h:36-52) */
+  static const int dx[8] = {1, -1, 0, 0, 1, -1, 1, -1}, dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+  if (!occ[(size_t)x * h + y]) return 0;
+  for (int k = 0; k < 8; ++k) {
+    int cx = x + dx[k], cy = y + dy[k];
+    if (0 <= cx && 0 <= cy && cx < w && cy < h && !occ[(size_t)cx * h + cy]) return 1;
+  }
+  return 0;
+}
+
+or_cddt* or_cddt_build(const uint8_t* occ, int w, int h, int td) {
+  or_cddt* c = (or_cddt*)calloc(1, sizeof(or_cddt));
+  c->td = td;
+  c->nang = 0;
+  while (c->nang < td && c->nang < td / 2.0) c->nang++; /* a < theta_discretization / 2.0 */
+  c->td_div_2pi = (float)((double)td / OR_M_2PI);              /* RangeLib.h:976 */
+  c->m2pi_div_td = (float)(OR_M_2PI / (double)(float)td);      /* :977 */
+  c->width = (int*)calloc(td, sizeof(int));
+  c->trans = (float*)calloc(td, sizeof(float));
+  c->cosv = (float*)calloc(td, sizeof(float));
+  c->sinv = (float*)calloc(td, sizeof(float));
+  c->bin_off = (long long*)calloc(td + 1, sizeof(long long));
+  for (int i = 0; i < td; ++i) {                                /* :991-1058 */
+    float angle = (float)i * c->m2pi_div_td;
+    float cs = cosf(angle), sn = sinf(angle);
+    c->cosv[i] = cs;
+    c->sinv[i] = sn;
+    float rotated_height = fabsf((float)w * sn) + fabsf((float)h * cs);
+    c->width[i] = (int)(unsigned)ceil((double)rotated_height - OR_EPS);
+    float lt = (float)h * cs, rt = (float)w * sn + (float)h * cs, rb = (float)w * sn;
+    float m2 = (rb < rt) ? rb : rt;                              /* std::min(rt, rb) */
+    float mn = (m2 < lt) ? m2 : lt;                              /* std::min(lt, ...) */
+    double t = -1.0 * (double)mn - OR_EPS;
+    c->trans[i] = (float)(t > 0.0 ? t : 0.0);                    /* std::max(0.0, t) */
+    c->bin_off[i + 1] = c->bin_off[i] + c->width[i];
+  }
+  const long long nb = c->bin_off[td];
+  long long* cnt = (long long*)calloc(nb + 1, sizeof(long long));
+  /* two passes over the edge cells (x-major, as the reference): count, then fill */
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int x = 0; x < w; ++x)
+      for (int y = 0; y < h; ++y) {
+        if (!edge_cell(occ, w, h, x, y)) continue;
+        float px = (float)(x + 0.5), py = (float)(y + 0.5);     /* :1098 */
+        for (int a = 0; a < c->nang; ++a) {
+          float cs = c->cosv[a], sn = c->sinv[a];
+          float half = (float)((double)(fabsf(sn) + fabsf(cs)) / 2.0);
+          float cx = px * cs - py * sn;
+          float cy = (px * sn + py * cs) + c->trans[a];
+          int upper = (int)((double)(cy + half) - OR_EPS);
+          int lower = (int)((double)(cy - half) + OR_EPS);
+          for (int i = lower; i <= upper; ++i) {
+            if (i < 0 || i >= c->width[a]) continue; /* outside the table: the reference writes out of bounds */
+            long long b = c->bin_off[a] + i;
+            if (pass == 0)
+              cnt[b + 1]++;
+            else
+              c->vals[cnt[b]++] = cx;
+          }
+        }
+      }
+    if (pass == 0) {
+      for (long long b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+      c->start = (long long*)malloc((nb + 1) * sizeof(long long));
+      memcpy(c->start, cnt, (nb + 1) * sizeof(long long));
+      c->vals = (float*)malloc((cnt[nb] > 0 ? cnt[nb] : 1) * sizeof(float));
+    }
+  }
+  free(cnt);
+  /* sort each bin, drop duplicates, compact (RangeLib.h:1131-1141) */
+  long long out = 0;
+  for (long long b = 0; b < nb; ++b) {
+    long long s = c->start[b], e = c->start[b + 1];
+    qsort(c->vals + s, (size_t)(e - s), sizeof(float), cmp_float);
+    c->start[b] = out;
+    for (long long k = s; k < e; ++k)
+      if (k == s || c->vals[k] != c->vals[k - 1]) c->vals[out++] = c->vals[k];
+  }
+  c->start[nb] = out;
+  return c;
+}
+
+void or_cddt_free(or_cddt* c) {
+  if (!c) return;
+  free(c->width);
+  free(c->trans);
+  free(c->cosv);
+  free(c->sinv);
+  free(c->bin_off);
+  free(c->start);
+  free(c->vals);
+  free(c);
+}
+
+/* first index with v[i] >= x (lower_bound) / v[i] > x (upper_bound) */
+static long long lower_b(const float* v, long long n, float x) {
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    long long mid = (lo + hi) / 2;
+    if (v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+static long long upper_b(const float* v, long long n, float x) {
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    long long mid = (lo + hi) / 2;
+    if (!(x < v[mid])) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+void or_cddt_prune(or_cddt* c, const uint8_t* occ, int w, int h, float max_range) {
+  /* CDDTCast::prune (RangeLib.h:1172-1275).  Where the reference's linear
+   * search leaves `index` unset (no entry >= the cell), the lower_bound value
+   * (the bin size) is taken, as the binary search would. */
+  const long long nb = c->bin_off[c->td];
+  uint8_t* used = (uint8_t*)calloc(c->start[nb] + 1, 1);
+  for (int a = 0; a < c->nang; ++a) {
+    float cs = c->cosv[a], sn = c->sinv[a], tr = c->trans[a];
+    for (int x = 0; x < w; ++x) {
+      float fx = (float)(0.5 + x); /* float _x = 0.5 + x (RangeLib.h:1205) */
+      for (int y = 0; y < h; ++y) {
+        float fy = (float)(0.5 + y);
+        float lx = fx * cs - fy * sn;
+        float ly = (fx * sn + fy * cs) + tr;
+        unsigned li = (unsigned)(int)ly;
+        if (li >= (unsigned)c->width[a]) continue; /* outside the table (the reference reads out of bounds) */
+        long long b = c->bin_off[a] + li, s = c->start[b];
+        long long n = c->start[b + 1] - s;
+        const float* v = c->vals + s;
+        long long high = n - 1;
+        if (high == -1) continue;
+        if (occ[(size_t)x * h + y]) continue;
+        if (v[high] < lx && lx - v[high] < max_range) {
+          used[s + high] = 1;
+          continue;
+        }
+        long long index = lower_b(v, n, lx); /* linear search (<= 64 entries) finds the same index */
+        if (index < n) used[s + index] = 1;
+        if (index - 1 >= 0) used[s + index - 1] = 1;
+      }
+    }
+  }
+  long long out = 0;
+  for (long long b = 0; b < nb; ++b) {
+    long long s = c->start[b], e = c->start[b + 1];
+    c->start[b] = out;
+    for (long long k = s; k < e; ++k)
+      if (used[k]) c->vals[out++] = c->vals[k];
+  }
+  c->start[nb] = out;
+  free(used);
+}
+
+static int cddt_discretize(const or_cddt* c, float theta, int* flipped) {
+  /* CDDTCast::discretize_theta (RangeLib.h:1279-1339); -1 where the wrap never ends */
+  if (theta != theta) return -1;
+  if (theta < 0.0) {
+    while (theta < 0.0) {
+      float next = (float)((double)theta + OR_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  } else if (theta > OR_M_2PI) {
+    while (theta > OR_M_2PI) {
+      float next = (float)((double)theta - OR_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  }
+  int fl = 0;
+  if (theta >= OR_M_PI) {
+    fl = 1;
+    theta = (float)((double)theta - OR_M_PI);
+  }
+  int rounded = (int)roundf(theta * c->td_div_2pi);
+  if (rounded == c->td >> 1) {
+    rounded = 0;
+    fl = !fl;
+  }
+  *flipped = fl;
+  return rounded % c->td;
+}
+
+float or_cddt_calc_range(const or_cddt* c, const uint8_t* occ, int w, int h, float max_range, float x, float y,
+                         float heading) {
+  /* CDDTCast::calc_range (RangeLib.h:1341-1514).  NaN where the reference's
+   * heading wrap never ends, and where it tests the occupancy of a cell
+   * outside the map (it indexes grid[x][y] with the raw coordinates). */
+  int fl = 0;
+  int a = cddt_discretize(c, (float)(-1.0 * (double)heading), &fl);
+  if (a < 0) return NAN;
+  float cs = c->cosv[a], sn = c->sinv[a];
+  float lx = x * cs - y * sn;
+  float ly = (x * sn + y * cs) + c->trans[a];
+  int li_i = (ly >= -2147483648.0f && ly < 2147483648.0f) ? (int)ly : INT_MIN;
+  unsigned li = (unsigned)li_i;
+  if (li >= (unsigned)c->width[a]) return max_range;
+  long long b = c->bin_off[a] + li, s = c->start[b];
+  long long n = c->start[b + 1] - s;
+  const float* v = c->vals + s;
+  long long high = n - 1;
+  if (high == -1) return max_range;
+  int cx = (x > -1.0f && x < (float)w) ? (int)x : -1, cy = (y > -1.0f && y < (float)h) ? (int)y : -1;
+  if (fl) {
+    if (v[0] > lx) return max_range;
+    if (v[high] < lx) return lx - v[high];
+    if (cx < 0 || cy < 0) return NAN;
+    if (occ[(size_t)cx * h + cy]) return 0.0f;
+    if (high > OR_BSEARCH) return lx - v[upper_b(v, n, lx) - 1];
+    for (long long i = high; i >= 0; --i)
+      if (v[i] <= lx) return lx - v[i];
+  } else {
+    if (v[high] < lx) return max_range;
+    if (v[0] > lx) return v[0] - lx;
+    if (cx < 0 || cy < 0) return NAN;
+    if (occ[(size_t)cx * h + cy]) return 0.0f;
+    if (high > OR_BSEARCH) {
+      long long idx = upper_b(v, n, lx);
+      if (idx == n) return 0.0f; /* v[high] == lx: the reference reads past the bin; the linear search gives 0 */
+      return v[idx] - lx;
+    }
+    for (long long i = 0; i < n; ++i)
+      if (v[i] >= lx) return v[i] - lx;
+  }
+  return -1.0f; /* not reached (assert(0) in the reference) */
 }

@@ -39,9 +39,23 @@ typedef struct {
   float scale, angle, origin_x, origin_y, sin_angle, cos_angle;
 } or_world;
 
+/* CDDTCast tables (RangeLib.h:1744-1749): angle a has width[a] bins; bin i of
+ * angle a holds vals[start[bin_off[a] + i] .. start[bin_off[a] + i + 1]). */
+typedef struct {
+  int td, nang;            /* theta discretization; angles that hold data */
+  int* width;              /* [td] */
+  float* trans;            /* [td] lut_translations */
+  float* cosv;             /* [td] cosf of each bin's angle */
+  float* sinv;             /* [td] */
+  long long* bin_off;      /* [td + 1] */
+  long long* start;        /* [bins + 1] */
+  float* vals;             /* sorted, duplicate-free obstacle positions per bin */
+  float td_div_2pi, m2pi_div_td;
+} or_cddt;
+
 /* One range method bound to a map (RangeMethod state, RangeLib.h:682-688). */
 typedef struct {
-  int kind;            /* 0 = RayMarching, 1 = BresenhamsLine, 2 = GiantLUTCast */
+  int kind;            /* 0 = RayMarching, 1 = BresenhamsLine, 2 = GiantLUTCast, 3 = CDDTCast */
   int width, height;   /* grid[x][y], x < width, y < height */
   const uint8_t* occ;  /* occupancy, x-major: occ[x*height + y] */
   const float* edt;    /* distance transform, x-major (RayMarching only) */
@@ -52,6 +66,7 @@ typedef struct {
   const uint16_t* glt; /* GiantLUTCast table, glt[(x*height + y)*glt_td + bin] */
   int glt_td;          /* theta discretization */
   float glt_max_div_limits, glt_td_div_2pi;
+  const or_cddt* cddt; /* CDDTCast tables */
 } or_method;
 
 /* DistanceTransform(OMap*) (RangeLib.h:345-373). */
@@ -80,6 +95,16 @@ int or_glt_discretize(float theta, int td, float td_div_2pi);
 /* GiantLUTCast::calc_range (RangeLib.h:1866-1877); NaN where or_glt_discretize is -1
  * and for a NaN origin (the reference indexes its table at (int)NaN). */
 float or_glt_calc_range(const or_method* m, float x, float y, float heading);
+
+/* CDDTCast(OMap, max_range, td) (RangeLib.h:974-1142); free with or_cddt_free. */
+or_cddt* or_cddt_build(const uint8_t* occ, int width, int height, int td);
+void or_cddt_free(or_cddt* c);
+/* CDDTCast::prune (RangeLib.h:1172-1275). */
+void or_cddt_prune(or_cddt* c, const uint8_t* occ, int width, int height, float max_range);
+/* CDDTCast::calc_range (RangeLib.h:1341-1514); NaN for a heading whose wrap never
+ * ends and for an occupancy test outside the map (out-of-bounds in the reference). */
+float or_cddt_calc_range(const or_cddt* c, const uint8_t* occ, int width, int height, float max_range, float x,
+                         float y, float heading);
 
 float or_calc_range(const or_method* m, float x, float y, float heading);
 

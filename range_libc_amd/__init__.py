@@ -18,11 +18,13 @@ import sys
 import numpy as np
 
 from . import _lib
-from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL, RL_METHOD_GLT, RL_METHOD_RM, Batch,
+from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL, RL_METHOD_CDDT, RL_METHOD_GLT,
+                   RL_METHOD_RM, Batch,
                    Context, DeviceMap, RangeLibError, Surrogate, default_context)
 
 __all__ = [
-    "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyGiantLUTCast", "PyNull",
+    "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyCDDTCast", "PyGiantLUTCast",
+    "PyNull",
     "RangeLibError", "Context", "DeviceMap", "Batch", "Surrogate", "default_context",
     "USE_CACHED_TRIG", "USE_ALTERNATE_MOD", "USE_CACHED_CONSTANTS", "USE_FAST_ROUND", "NO_INLINE",
     "USE_LRU_CACHE", "LRU_CACHE_SIZE", "SHOULD_USE_CUDA",
@@ -242,6 +244,25 @@ class PyBresenhamsLine(_GpuMethod):
     """BresenhamsLine (RangeLib.h:691-772; RangeLibc.pyx:200-225), run on the GPU."""
 
     METHOD = RL_METHOD_BL
+
+    def calc_range(self, x, y, heading):
+        return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))
+
+
+class PyCDDTCast(_GpuMethod):
+    """CDDTCast (RangeLib.h:971-1770; RangeLibc.pyx:253-287): compressed directional
+    distance transform, tables built on the GPU at construction; prune() makes it PCDDT."""
+
+    METHOD = RL_METHOD_CDDT
+
+    def __init__(self, Map: PyOMap, max_range: float, theta_disc: int, device: int = 0):
+        super().__init__(Map, max_range, device)
+        self.theta_disc = int(theta_disc)
+        self.map.build_cddt(self.theta_disc)
+
+    def prune(self, max_range: float = -1.0):
+        """RangeLibc.pyx:261-265: a negative max_range prunes at the constructor's."""
+        self.map.prune_cddt(self.max_range if max_range < 0.0 else float(np.float32(max_range)))
 
     def calc_range(self, x, y, heading):
         return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))

@@ -17,6 +17,7 @@ LIB_PATH = os.environ.get("RL_LIBRARY", os.path.join(PKG, "librangelib_hip.so"))
 RL_METHOD_RM = 0
 RL_METHOD_BL = 1
 RL_METHOD_GLT = 2
+RL_METHOD_CDDT = 3
 RL_FRAME_GRID = 0
 RL_FRAME_WORLD = 1
 RL_FRAME_ANGLES = 2
@@ -43,6 +44,10 @@ SIGNATURES = {
     "rl_map_create_from_edt": [_vp, _u8p, _fp, C.c_int, C.c_int, C.c_uint, C.POINTER(_vp)],
     "rl_map_destroy": [_vp],
     "rl_map_build_giant_lut": [_vp, C.c_float, C.c_int],
+    "rl_map_build_cddt": [_vp, C.c_int],
+    "rl_map_prune_cddt": [_vp, C.c_float],
+    "rl_map_get_cddt": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _vp, _vp, _vp,
+                        _vp, _fp],
     "rl_map_get_giant_lut": [_vp, _vp, C.POINTER(C.c_int), _fp, _fp],
     "rl_map_set_world": [_vp] + [C.c_float] * 6,
     "rl_map_set_sensor_table": [_vp, _dp, C.c_int],
@@ -248,6 +253,28 @@ class DeviceMap:
             out = np.empty((self.width, self.height, td.value), dtype=np.uint16)
             check(lib().rl_map_get_giant_lut(self.handle, C.c_void_p(out.ctypes.data), None, None, None))
         return out, td.value, r.value, ms.value
+
+    def build_cddt(self, theta_disc: int):
+        """CDDTCast tables of this map (RL_METHOD_CDDT then works)."""
+        check(lib().rl_map_build_cddt(self.handle, int(theta_disc)))
+
+    def prune_cddt(self, max_range: float):
+        """CDDTCast::prune: keep only the entries a query can return (PCDDT)."""
+        check(lib().rl_map_prune_cddt(self.handle, max_range))
+
+    def cddt_tables(self) -> dict:
+        """CSR copy of the tables: widths, trans, start, vals, build_ms."""
+        td, nb, nv, ms = C.c_int(), C.c_longlong(), C.c_longlong(), C.c_float()
+        check(lib().rl_map_get_cddt(self.handle, C.byref(td), C.byref(nb), C.byref(nv), None, None, None, None,
+                                    C.byref(ms)))
+        widths = np.empty(td.value, np.int32)
+        trans = np.empty(td.value, np.float32)
+        start = np.empty(nb.value + 1, np.int64)
+        vals = np.empty(max(nv.value, 1), np.float32)
+        check(lib().rl_map_get_cddt(self.handle, None, None, None, C.c_void_p(widths.ctypes.data),
+                                    C.c_void_p(trans.ctypes.data), C.c_void_p(start.ctypes.data),
+                                    C.c_void_p(vals.ctypes.data), None))
+        return {"widths": widths, "trans": trans, "start": start, "vals": vals[: nv.value], "build_ms": ms.value}
 
     def edt(self) -> np.ndarray:
         out = np.empty((self.width, self.height), dtype=np.float32)

@@ -163,6 +163,9 @@ int batch_launch_locked(rl_batch* b) {
   else if (b->method == RL_METHOD_GLT)
     hipLaunchKernelGGL(k_batch<RL_METHOD_GLT>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
                        b->n_items, c->trig, b->lds_box);
+  else if (b->method == RL_METHOD_CDDT)
+    hipLaunchKernelGGL(k_batch<RL_METHOD_CDDT>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
+                       b->n_items, c->trig, b->lds_box);
   else
     hipLaunchKernelGGL(k_batch<RL_METHOD_RM>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
                        b->n_items, c->trig, b->lds_box);

@@ -49,6 +49,22 @@ struct WorldDev {
 #define RL_IDX8_ESCAPE 255u
 #define RL_LUT_SIZE 256
 
+// CDDTCast tables (built by rl_map_build_cddt): angle a has width[a] bins;
+// bin i of angle a holds vals[start[bin_off[a] + i] .. start[bin_off[a] + i + 1]),
+// sorted and free of duplicates; cosv / sinv / trans are the reference's
+// per-angle cosf, sinf and lut_translations.
+struct CddtDev {
+  const int* width;
+  const float* trans;
+  const float* cosv;
+  const float* sinv;
+  const long long* bin_off;
+  const long long* start;
+  const float* vals;
+  int td;
+  float td_div_2pi;
+};
+
 struct MapDev {
   const float* edt;     // x-major W*H
   const uint8_t* occ;   // x-major W*H
@@ -62,6 +78,7 @@ struct MapDev {
   const uint16_t* glt;
   int glt_td;
   float glt_R, glt_scale, glt_td_div_2pi;  // its max_range, max_div_limits, td / M_2PI
+  CddtDev cddt;  // CDDTCast tables (null pointers when not built)
 };
 
 __device__ __forceinline__ void sincos_libc(float h, const TrigDev& t, float* s, float* c) {
@@ -313,6 +330,106 @@ __device__ __forceinline__ float glt_calc_range(const MapDev& m, float x, float 
   if (COUNT) *reads = 1;
   const size_t cell = (size_t)(int)x * (unsigned)m.height + (unsigned)(int)y;
   return (float)glb(m.glt)[cell * (unsigned)m.glt_td + (unsigned)b] * m.glt_scale;
+}
+
+// CDDTCast::discretize_theta (RangeLib.h:1279-1339): the GiantLUTCast wrap,
+// then the half-turn symmetry (theta >= M_PI: flipped, theta -= M_PI in
+// double), roundf(theta * (td / M_2PI)), and a bin that rounds up to td / 2
+// wraps to 0 with the flip toggled.  -1 where the wrap never ends.
+#define RL_M_PI 3.14159265358979323846
+__device__ __forceinline__ int cddt_bin(const CddtDev& c, float theta, bool* flipped) {
+  if (theta != theta) return -1;
+  if (theta < 0.0) {
+    while (theta < 0.0) {
+      const float next = (float)((double)theta + RL_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  } else if (theta > RL_M_2PI) {
+    while (theta > RL_M_2PI) {
+      const float next = (float)((double)theta - RL_M_2PI);
+      if (next == theta) return -1;
+      theta = next;
+    }
+  }
+  bool fl = false;
+  if (theta >= RL_M_PI) {
+    fl = true;
+    theta = (float)((double)theta - RL_M_PI);
+  }
+  int rounded = (int)roundf(theta * c.td_div_2pi);
+  if (rounded == c.td >> 1) {
+    rounded = 0;
+    fl = !fl;
+  }
+  *flipped = fl;
+  return rounded % c.td;
+}
+
+// CDDTCast::calc_range (RangeLib.h:1341-1514): the origin is projected onto
+// the bin of the (discretized) heading and the nearest obstacle position
+// ahead of it (behind it when flipped) is searched in that bin: upper_bound
+// above 64 entries, a linear scan below, as the reference.  NaN where the
+// heading's wrap never ends and where the reference tests the occupancy of a
+// cell outside the map (it indexes grid[x][y] out of bounds); 0 where its
+// binary search would read past the bin (the query projects exactly onto the
+// bin's last entry), which is what its linear scan returns.
+template <bool COUNT>
+__device__ __forceinline__ float cddt_calc_range(const MapDev& m, float max_range, float x, float y, float heading,
+                                                 int* reads) {
+  const CddtDev& c = m.cddt;
+  if (COUNT) *reads = 0;
+  bool fl;
+  const int a = cddt_bin(c, -heading, &fl);
+  if (a < 0) return __builtin_nanf("");
+  const float cs = c.cosv[a], sn = c.sinv[a];
+  const float lx = x * cs - y * sn;
+  const float ly = (x * sn + y * cs) + c.trans[a];
+  const unsigned li = (unsigned)trunc_x86(ly);
+  if (li >= (unsigned)c.width[a]) return max_range;
+  const long long b = c.bin_off[a] + li, s = c.start[b];
+  const long long n = c.start[b + 1] - s;
+  const RL_GLB float* v = glb(c.vals) + s;
+  const long long high = n - 1;
+  if (high == -1) return max_range;
+  if (COUNT) *reads = 1;
+  const bool inside = x > -1.0f && x < (float)m.width && y > -1.0f && y < (float)m.height;
+  if (fl) {
+    if (v[0] > lx) return max_range;
+    if (v[high] < lx) return lx - v[high];
+    if (!inside) return __builtin_nanf("");
+    if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) return 0.0f;
+    if (high > 64) {
+      long long lo = 0, hi = n;  // upper_bound
+      while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
+      }
+      return lx - v[lo - 1];
+    }
+    for (long long i = high; i >= 0; --i) {
+      const float o = v[i];
+      if (o <= lx) return lx - o;
+    }
+  } else {
+    if (v[high] < lx) return max_range;
+    if (v[0] > lx) return v[0] - lx;
+    if (!inside) return __builtin_nanf("");
+    if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) return 0.0f;
+    if (high > 64) {
+      long long lo = 0, hi = n;
+      while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
+      }
+      return lo == n ? 0.0f : v[lo] - lx;
+    }
+    for (long long i = 0; i < n; ++i) {
+      const float o = v[i];
+      if (o >= lx) return o - lx;
+    }
+  }
+  return -1.0f;  // not reached: the checks above bracket lx within the bin
 }
 
 __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_range) {

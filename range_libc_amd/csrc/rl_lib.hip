@@ -181,6 +181,7 @@ __device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, con
                                           const TrigDev& trig, float max_range, float x, float y,
                                           float heading, int* cnt) {
   if (METHOD == RL_METHOD_GLT) return rl::glt_calc_range<COUNT>(m, x, y, heading, cnt);
+  if (METHOD == RL_METHOD_CDDT) return rl::cddt_calc_range<COUNT>(m, max_range, x, y, heading, cnt);
   float s, c;
   rl::sincos_libc(heading, trig, &s, &c);
   if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
@@ -682,10 +683,19 @@ struct rl_map {
   uint16_t* d_glt = nullptr;  // GiantLUTCast table (rl_map_build_giant_lut)
   int glt_td = 0;
   float glt_R = 0.0f, glt_scale = 0.0f, glt_td_div_2pi = 0.0f, glt_ms = 0.0f;
+  // CDDTCast tables (rl_map_build_cddt; layout in rl_device.h)
+  int* cddt_width = nullptr;
+  float *cddt_trans = nullptr, *cddt_cos = nullptr, *cddt_sin = nullptr, *cddt_vals = nullptr;
+  long long *cddt_bin_off = nullptr, *cddt_start = nullptr;
+  int cddt_td = 0, cddt_nang = 0;
+  long long cddt_nb = 0, cddt_nv = 0;
+  float cddt_td_div_2pi = 0.0f, cddt_ms = 0.0f;
 
   MapDev dev() const {
     return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut, width, height, nby, ctx->lds_box, world,
-                  d_glt, glt_td, glt_R, glt_scale, glt_td_div_2pi};
+                  d_glt, glt_td, glt_R, glt_scale, glt_td_div_2pi,
+                  rl::CddtDev{cddt_width, cddt_trans, cddt_cos, cddt_sin, cddt_bin_off, cddt_start, cddt_vals,
+                              cddt_td, cddt_td_div_2pi}};
   }
 };
 
@@ -703,7 +713,7 @@ void set_world(rl_map* m, float scale, float angle, float ox, float oy, float si
 }
 
 int check_method(int method) {
-  if (method != RL_METHOD_RM && method != RL_METHOD_BL && method != RL_METHOD_GLT)
+  if (method < RL_METHOD_RM || method > RL_METHOD_CDDT)
     return fail(RL_ERR_ARG, "unknown range method %d", method);
   return RL_OK;
 }
@@ -737,6 +747,8 @@ int launch_cast_f(rl_map* m, int method, float max_range, const float* ins, cons
     return launch_cast_t<RL_METHOD_BL, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
   if (method == RL_METHOD_GLT)
     return launch_cast_t<RL_METHOD_GLT, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
+  if (method == RL_METHOD_CDDT)
+    return launch_cast_t<RL_METHOD_CDDT, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
   return launch_cast_t<RL_METHOD_RM, FRAME, COUNT>(m, max_range, ins, angles, outs, counts, n_rays, num_angles);
 }
 
@@ -785,6 +797,9 @@ int launch_fused(rl_map* m, int method, float max_range, const float* ins, const
                        max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
   else if (method == RL_METHOD_GLT)
     hipLaunchKernelGGL(k_fused<RL_METHOD_GLT>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
+                       max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
+  else if (method == RL_METHOD_CDDT)
+    hipLaunchKernelGGL(k_fused<RL_METHOD_CDDT>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
                        max_range, ins, angles, obs, m->d_table, m->table_width, weights, N, M, S);
   else
     hipLaunchKernelGGL(k_fused<RL_METHOD_RM>, grid, block, 0, m->ctx->stream, m->dev(), m->ctx->trig,
@@ -848,6 +863,9 @@ int launch_particles(rl_map* m, int method, bool sensor, float max_range, const 
   if (method == RL_METHOD_GLT)
     return sensor ? launch_particles_t<RL_METHOD_GLT, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
                   : launch_particles_t<RL_METHOD_GLT, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
+  if (method == RL_METHOD_CDDT)
+    return sensor ? launch_particles_t<RL_METHOD_CDDT, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
+                  : launch_particles_t<RL_METHOD_CDDT, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
   return sensor ? launch_particles_t<RL_METHOD_RM, true>(m, max_range, ins, angles, obs, ranges, weights, N, M)
                 : launch_particles_t<RL_METHOD_RM, false>(m, max_range, ins, angles, obs, ranges, weights, N, M);
 }
@@ -931,6 +949,8 @@ int build_edt(rl_map* m, unsigned flags) {
 int check_map(const rl_map* m, int method, float max_range) {
   TRY(check_method(method));
   if (method == RL_METHOD_RM && !m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
+  if (method == RL_METHOD_CDDT && !m->cddt_vals)
+    return fail(RL_ERR_STATE, "no CDDT tables (call rl_map_build_cddt first)");
   if (method == RL_METHOD_GLT) {
     if (!m->d_glt) return fail(RL_ERR_STATE, "no GiantLUTCast table (call rl_map_build_giant_lut first)");
     if (!(max_range == m->glt_R))
@@ -1201,6 +1221,12 @@ int rl_map_destroy(rl_map* m) {
   if (m->d_sched) (void)hipFree(m->d_sched);
   if (m->d_table) (void)hipFree(m->d_table);
   if (m->d_glt) (void)hipFree(m->d_glt);
+  {
+    void* ps[] = {m->cddt_width, m->cddt_trans, m->cddt_cos, m->cddt_sin, m->cddt_vals, m->cddt_bin_off,
+                  m->cddt_start};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
   delete m;
   return rl_destroy(c);  // drops the map's reference
 }
@@ -1466,3 +1492,6 @@ int rl_debug_sqrtf_dev(rl_ctx* c, const float* x, float* out, int n) {
 #include <cmath>
 #include <random>
 #include "rl_surrogate_impl.h"
+
+// CDDTCast / PCDDT tables built on the GPU (uses the types above).
+#include "rl_cddt_impl.h"

@@ -168,6 +168,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "degenerate.npz"), **d)
 
     giant_lut()
+    cddt()
     listing()
 
 
@@ -212,6 +213,61 @@ def giant_lut():
     np.savez_compressed(os.path.join(HERE, "giant_lut.npz"), **g)
 
 
+def cddt_cases():
+    """(name, image, max_range, theta_discretization) of the CDDTCast fixtures."""
+    return [("rect64", W.rect_disk_map(64, seed=3), 60.0, 108),
+            ("corr96x80", np.ascontiguousarray(W.corridor_map(96, seed=2)[:, :80]), 120.0, 37),
+            ("corr300", W.corridor_map(300, seed=4), 200.0, 120)]
+
+
+def cddt():
+    """CDDTCast and its pruned form built and queried by the compiled reference:
+    table hashes (widths, translations, bin offsets, values) and queries from
+    inside the map."""
+    g = {}
+    world = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+    for name, img, R, td in cddt_cases():
+        h, w = img.shape
+        g[f"{name}_img"] = img
+        rng = np.random.default_rng(5)
+        n = 3000
+        q = np.stack([rng.uniform(0, w, n), rng.uniform(0, h, n),
+                      np.concatenate([rng.uniform(-30, 30, n // 2), rng.uniform(-3000, 3000, n - n // 2)])],
+                     axis=1).astype(np.float32)
+        g[f"{name}_ins"] = q
+        for tag, prune in (("full", False), ("pruned", True)):
+            ref = O.Reference(img, R, "cddt", theta_disc=td)
+            if prune:
+                ref.cddt_prune(R)
+            t = ref.cddt_tables()
+            for k in ("widths", "trans", "start", "vals"):
+                g[f"{name}_{tag}_{k}_sha"] = np.array(sha(t[k]))
+            g[f"{name}_{tag}_nvals"] = np.array(len(t["vals"]))
+            g[f"{name}_{tag}_grid"] = ref.calc_range_many(q)
+        # world-frame queries land outside the map for this world: keep those inside
+        refw = O.Reference(img, R, "cddt", world=world, theta_disc=td)
+        g[f"{name}_world_rot"] = refw.numpy_calc_range(world_inside(q, world, w, h))
+    np.savez_compressed(os.path.join(HERE, "cddt.npz"), **g)
+
+
+def world_inside(q, world, w, h):
+    """World poses whose grid origin lies inside the map, half a cell from its
+    edges.  The reference's transform (RangeLib.h:463-475) rotates
+    (x, y) = ((xw - ox) / s, (yw - oy) / s) by the map angle and marches from
+    (y', x'); this inverts it for origins (q[:, 0], q[:, 1]) scaled into
+    [0.5, w - 0.5) x [0.5, h - 0.5)."""
+    scale, ang, ox, oy, s, c = world
+    gx = 0.5 + q[:, 0].astype(np.float64) / w * (w - 1.0)  # march origin, first grid index
+    gy = 0.5 + q[:, 1].astype(np.float64) / h * (h - 1.0)
+    xr, yr = gy, gx  # (x', y') with calc_range(y', x', .)
+    x = c * xr + s * yr
+    y = -s * xr + c * yr
+    out = q.copy()
+    out[:, 0] = (x * scale + ox).astype(np.float32)
+    out[:, 1] = (y * scale + oy).astype(np.float32)
+    return out
+
+
 def listing():
     for f in sorted(os.listdir(HERE)):
         p = os.path.join(HERE, f)
@@ -222,6 +278,9 @@ def listing():
 if __name__ == "__main__":
     if sys.argv[1:] == ["glt"]:
         giant_lut()
+        listing()
+    elif sys.argv[1:] == ["cddt"]:
+        cddt()
         listing()
     else:
         main()

@@ -1,0 +1,151 @@
+"""CDDTCast / PCDDT on the GPU: tables built and pruned on the device against the
+compiled reference's fixture and the oracle, and every casting call with
+RL_METHOD_CDDT bit-exact against the oracle."""
+import hashlib
+import math
+import os
+
+import numpy as np
+import pytest
+
+import range_libc_amd as rl
+from oracle import oracle as O
+from range_libc_amd import workloads as W
+from tests.conftest import GOLDEN, assert_bits_equal
+from tests.golden.make_golden import world_inside
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"rect64": (60.0, 108), "corr96x80": (120.0, 37), "corr300": (200.0, 120)}
+WORLD = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU test needs a device"
+    return torch
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(GOLDEN, "cddt.npz"))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cddt_tables_and_queries_golden(ctx, golden, name):
+    R, td = CASES[name]
+    img = golden[f"{name}_img"]
+    h, w = img.shape
+    q = golden[f"{name}_ins"]
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    dm.build_cddt(td)
+    for tag in ("full", "pruned"):
+        if tag == "pruned":
+            dm.prune_cddt(R)
+        t = dm.cddt_tables()
+        assert _sha(t["widths"]) == str(golden[f"{name}_{tag}_widths_sha"])
+        assert _sha(t["trans"]) == str(golden[f"{name}_{tag}_trans_sha"])
+        assert _sha(t["start"]) == str(golden[f"{name}_{tag}_start_sha"])
+        assert _sha(t["vals"]) == str(golden[f"{name}_{tag}_vals_sha"]), f"{tag}: {len(t['vals'])} values"
+        out = np.empty(q.shape[0], np.float32)
+        dm.calc_range_many(rl.RL_METHOD_CDDT, R, q, out, q.shape[0])
+        assert_bits_equal(out, golden[f"{name}_{tag}_grid"], f"{tag} grid")
+    full = rl.DeviceMap(ctx, O.xmajor(img))
+    full.build_cddt(td)
+    full.set_world(*WORLD)
+    qw = world_inside(q, WORLD, w, h)
+    out = np.empty(q.shape[0], np.float32)
+    full.numpy_calc_range(rl.RL_METHOD_CDDT, R, qw, out, q.shape[0])
+    assert_bits_equal(out, golden[f"{name}_world_rot"], "rotated world")
+
+
+def test_cddt_particles_sensor_and_batch(ctx):
+    torch = _torch()
+    img = W.corridor_map(220, seed=9)
+    R, td = 180.0, 90
+    table = W.sensor_table(181)
+    o = O.Oracle(img, R, "cddt", theta_disc=td, prune=True)
+    o.set_sensor_model(table)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    dm.build_cddt(td)
+    dm.prune_cddt(R)
+    dm.set_sensor_table(table)
+    ins = W.world_poses(img, 400, seed=3)
+    ang = W.beam_angles(60)
+    want_r = o.numpy_calc_range_angles(ins, ang)
+    obs = W.noisy_observation(want_r[:60], seed=2, max_range=R)
+    want_w = o.eval_sensor_model(obs, want_r, 60, 400)
+    d = {k: torch.from_numpy(v).cuda() for k, v in (("ins", ins), ("ang", ang), ("obs", obs))}
+    rng_ = torch.empty(400 * 60, dtype=torch.float32, device="cuda")
+    wd = torch.empty(400, dtype=torch.float64, device="cuda")
+    dm.pf_step_dev(rl.RL_METHOD_CDDT, R, d["ins"].data_ptr(), d["ang"].data_ptr(), d["obs"].data_ptr(),
+                   rng_.data_ptr(), wd.data_ptr(), 400, 60)
+    ctx.sync()
+    assert_bits_equal(rng_.cpu().numpy(), want_r, "pf_step ranges")
+    assert np.array_equal(wd.cpu().numpy(), want_w)
+    w = np.empty(400, np.float64)
+    dm.repeat_angles_eval_sensor_model(rl.RL_METHOD_CDDT, R, ins, ang, obs, w, 400, 60)
+    assert np.array_equal(w, o.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))
+    # the multi-map batch over two CDDT maps
+    img2 = W.rect_disk_map(150, seed=4)
+    dm2 = rl.DeviceMap(ctx, O.xmajor(img2))
+    dm2.build_cddt(50)
+    o2 = O.Oracle(img2, 90.0, "cddt", theta_disc=50)
+    ins2 = W.world_poses(img2, 77, seed=5)
+    ang2 = W.beam_angles(33)
+    d2 = {k: torch.from_numpy(v).cuda() for k, v in (("ins", ins2), ("ang", ang2))}
+    out1 = torch.full((400 * 60,), -1.0, dtype=torch.float32, device="cuda")
+    out2 = torch.full((77 * 33,), -1.0, dtype=torch.float32, device="cuda")
+    b = rl.Batch(ctx, rl.RL_METHOD_CDDT)
+    b.set_jobs([{"map": dm, "max_range": R, "ins": d["ins"].data_ptr(), "angles": d["ang"].data_ptr(),
+                 "ranges": out1.data_ptr(), "num_particles": 400, "num_angles": 60},
+                {"map": dm2, "max_range": 90.0, "ins": d2["ins"].data_ptr(), "angles": d2["ang"].data_ptr(),
+                 "ranges": out2.data_ptr(), "num_particles": 77, "num_angles": 33}])
+    b.launch()
+    ctx.sync()
+    assert_bits_equal(out1.cpu().numpy(), want_r, "batch job 0")
+    assert_bits_equal(out2.cpu().numpy(), o2.numpy_calc_range_angles(ins2, ang2), "batch job 1")
+
+
+def test_py_cddt_cast(ctx):
+    img = W.rect_disk_map(90, seed=6)
+    omap = rl.PyOMap(img)
+    c = rl.PyCDDTCast(omap, 80.0, 72)
+    o = O.Oracle(img, 80.0, "cddt", theta_disc=72)
+    ins = W.world_poses(img, 200, seed=7)
+    out = np.empty(200, np.float32)
+    c.calc_range_many(ins, out)
+    assert_bits_equal(out, o.numpy_calc_range(ins), "full")
+    c.prune()
+    op = O.Oracle(img, 80.0, "cddt", theta_disc=72, prune=True)
+    c.calc_range_many(ins, out)
+    assert_bits_equal(out, op.numpy_calc_range(ins), "pruned")
+    assert c.calc_range(20.5, 30.25, 0.7) == op.calc_range(20.5, 30.25, 0.7)
+
+
+def test_cddt_special_cases_and_errors(ctx):
+    img = W.rect_disk_map(60, seed=2)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    q = np.array([[10.5, 10.5, 1.0]], np.float32)
+    out = np.empty(1, np.float32)
+    with pytest.raises(rl.RangeLibError):
+        dm.calc_range_many(rl.RL_METHOD_CDDT, 50.0, q, out, 1)
+    with pytest.raises(rl.RangeLibError):
+        dm.prune_cddt(50.0)
+    dm.build_cddt(40)
+    o = O.Oracle(img, 50.0, "cddt", theta_disc=40)
+    rng = np.random.default_rng(0)
+    heads = np.array([float("nan"), 1e10, 3e4, -3e4, 0.0, -0.0, math.pi, -math.pi, 2 * math.pi], np.float32)
+    qs = np.stack([rng.uniform(-20, 80, heads.size), rng.uniform(-20, 80, heads.size), heads], 1).astype(np.float32)
+    qs = np.concatenate([qs, np.array([[float("nan"), 3.0, 1.0], [-5.0, 30.0, 0.3], [30.0, 75.0, 2.0]], np.float32)])
+    out = np.empty(qs.shape[0], np.float32)
+    dm.calc_range_many(rl.RL_METHOD_CDDT, 50.0, qs, out, qs.shape[0])
+    assert_bits_equal(out, o.calc_range_many(qs), "special cases")  # NaN patterns included
+    dm.build_cddt(1)  # one bin: rebuilt in place
+    t = dm.cddt_tables()
+    assert t["widths"].shape == (1,)

@@ -388,3 +388,78 @@ def test_glt_oracle_matches_compiled_reference(seed):
                   rng.uniform(-100, 100, 3000)], 1).astype(np.float32)
     assert_bits_equal(o.calc_range_many(q), r.calc_range_many(q), "grid")
     assert_bits_equal(o.numpy_calc_range(q), r.numpy_calc_range(q), "world")
+
+
+# --------------------------------------------------------------------------- CDDTCast / PCDDT
+
+CDDT_CASES = {"rect64": (60.0, 108), "corr96x80": (120.0, 37), "corr300": (200.0, 120)}
+CDDT_WORLD = (0.05, 0.3, -3.0, 2.0, math.sin(0.3), math.cos(0.3))
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", sorted(CDDT_CASES))
+def test_cddt_oracle_golden(name):
+    """Tables (full and pruned) and queries of the compiled reference's CDDTCast
+    equal the oracle's."""
+    from tests.golden.make_golden import world_inside
+
+    g = np.load(os.path.join(GOLDEN, "cddt.npz"))
+    R, td = CDDT_CASES[name]
+    img = g[f"{name}_img"]
+    q = g[f"{name}_ins"]
+    for tag, prune in (("full", False), ("pruned", True)):
+        o = O.Oracle(img, R, "cddt", theta_disc=td, prune=prune)
+        t = o.cddt_tables()
+        assert t["widths"].astype(np.int32).tobytes() and _sha(t["widths"].astype(np.int32)) == str(g[f"{name}_{tag}_widths_sha"])
+        assert _sha(t["trans"]) == str(g[f"{name}_{tag}_trans_sha"])
+        assert _sha(t["start"].astype(np.int64)) == str(g[f"{name}_{tag}_start_sha"])
+        assert _sha(t["vals"]) == str(g[f"{name}_{tag}_vals_sha"])
+        assert_bits_equal(o.calc_range_many(q), g[f"{name}_{tag}_grid"], f"{name} {tag} grid")
+    h, w = img.shape
+    ow = O.Oracle(img, R, "cddt", CDDT_WORLD, theta_disc=td)
+    assert_bits_equal(ow.numpy_calc_range(world_inside(q, CDDT_WORLD, w, h)), g[f"{name}_world_rot"],
+                      f"{name} rotated world")
+
+
+def test_cddt_pruning_keeps_every_answer():
+    """Pruning removes table entries but no query from a free cell centre changes
+    (the entries it drops are never returned), and the pruned table is smaller."""
+    img = W.corridor_map(160, seed=8)
+    R, td = 150.0, 64
+    full = O.Oracle(img, R, "cddt", theta_disc=td)
+    pruned = O.Oracle(img, R, "cddt", theta_disc=td, prune=True)
+    assert 0 < len(pruned.cddt_tables()["vals"]) < len(full.cddt_tables()["vals"])
+    cells = W.free_cells(img)[::7].astype(np.float32) + 0.5
+    step = np.float32(2 * math.pi / td)
+    for i in range(0, td, 5):
+        q = np.concatenate([cells, np.full((len(cells), 1), np.float32(i) * step)], axis=1).astype(np.float32)
+        a, b = full.calc_range_many(q), pruned.calc_range_many(q)
+        ok = a < np.float32(R)  # a miss beyond max_range may hit a dropped entry; those return max_range unpruned
+        assert_bits_equal(a[ok], b[ok], f"bin {i}")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(3))
+def test_cddt_oracle_matches_compiled_reference(seed):
+    rng = np.random.default_rng(3000 + seed)
+    h, w = int(rng.integers(8, 70)), int(rng.integers(8, 70))
+    img = (rng.random((h, w)) < rng.uniform(0.02, 0.3)).astype(np.uint8)
+    img[0, :] = img[-1, :] = 1
+    img[:, 0] = img[:, -1] = 1
+    R, td = float(rng.uniform(5, 200)), int(rng.integers(2, 200))
+    for prune in (False, True):
+        o = O.Oracle(img, R, "cddt", theta_disc=td, prune=prune)
+        r = O.Reference(img, R, "cddt", theta_disc=td)
+        if prune:
+            r.cddt_prune(R)
+        a, b = o.cddt_tables(), r.cddt_tables()
+        for k in ("widths", "start"):
+            assert np.array_equal(a[k], b[k]), k
+        assert_bits_equal(a["trans"], b["trans"], "trans")
+        assert_bits_equal(a["vals"], b["vals"], "vals")
+        q = np.stack([rng.uniform(0, w, 3000), rng.uniform(0, h, 3000), rng.uniform(-100, 100, 3000)],
+                     1).astype(np.float32)
+        assert_bits_equal(o.calc_range_many(q), r.calc_range_many(q), "grid")
