@@ -307,6 +307,82 @@ def bench_glt(args):
     print(json.dumps(result))
 
 
+def bench_cddt(args):
+    """Secondary workload: CDDTCast / PCDDT on the particle-filter step's map
+    (2048x2048 corridors, max_range 500, 108 heading bins): tables built and
+    pruned on the GPU, the reference's CPU constructor and prune timed on a
+    bounded sample (a 512x512 corridor map), and the particle-filter step
+    answered from the pruned tables."""
+    import torch
+
+    import range_libc_amd as rl
+    from range_libc_amd import workloads as W
+
+    TD = 108
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    img, ins, angles, table, true_pose, seed_noise = build_workload()
+    m = rl.DeviceMap(ctx, np.ascontiguousarray(img.T.astype(np.uint8)))
+    m.set_sensor_table(table)
+    builds, prunes = [], []
+    for _ in range(3):
+        m.build_cddt(TD)
+        builds.append(m.cddt_tables()["build_ms"])
+        t0 = time.perf_counter()
+        m.prune_cddt(MAX_RANGE)
+        prunes.append((time.perf_counter() - t0) * 1e3)
+    t = m.cddt_tables()
+    d_ins = torch.from_numpy(ins).to(dev)
+    d_ang = torch.from_numpy(angles).to(dev)
+    d_obs = torch.from_numpy(W.noisy_observation(np.full(BEAMS, 50.0, np.float32), seed_noise,
+                                                 max_range=MAX_RANGE)).to(dev)
+    d_rng = torch.empty(PARTICLES * BEAMS, dtype=torch.float32, device=dev)
+    d_w = torch.empty(PARTICLES, dtype=torch.float64, device=dev)
+
+    def step(method):
+        m.pf_step_dev(method, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), d_obs.data_ptr(), d_rng.data_ptr(),
+                      d_w.data_ptr(), PARTICLES, BEAMS)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda._sleep(50_000_000)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    cddt_ms = timed(lambda: step(rl.RL_METHOD_CDDT), args.steps)
+    rm_ms = timed(lambda: step(rl.RL_METHOD_RM), args.steps)
+    result = {
+        "metric": "PCDDT tables built and pruned on 1 GPU (2048^2 corridors, 108 bins, R=500), ms",
+        "value": float(np.median(builds)) + float(np.median(prunes)), "unit": "ms", "higher_is_better": False,
+        "build_ms": float(np.median(builds)), "prune_ms_wall": float(np.median(prunes)),
+        "pruned_entries": int(len(t["vals"])), "bins": int(len(t["start"]) - 1),
+        "pf_step_pcddt_ms": cddt_ms, "pf_step_rm_ms": rm_ms,
+        "pf_step_pcddt_rays_per_s": PARTICLES * BEAMS / (cddt_ms * 1e-3),
+        "config": {"workload": "pcddt_corridor2048_td108_r500"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        if O.ref_available():
+            small = W.corridor_map(512, 1)
+            t0 = time.perf_counter()
+            ref = O.Reference(small, MAX_RANGE, "cddt", theta_disc=TD)
+            t1 = time.perf_counter()
+            ref.cddt_prune(MAX_RANGE)
+            t2 = time.perf_counter()
+            result["cpu_baseline"] = {"value": (t2 - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "reference",
+                                      "sample": f"CDDTCast constructor ({t1 - t0:.2f} s) + prune ({t2 - t1:.2f} s) "
+                                                f"on a 512x512 corridor map, {TD} bins (1/16 of the cells)"}
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,7 +392,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
-    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt"],
+    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt", "cddt"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
                          "surrogate (learned start point on the step's map)")
     ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
@@ -338,6 +414,8 @@ def main():
         return bench_surrogate(args)
     if args.workload == "glt":
         return bench_glt(args)
+    if args.workload == "cddt":
+        return bench_cddt(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)

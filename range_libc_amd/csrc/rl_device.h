@@ -368,8 +368,11 @@ __device__ __forceinline__ int cddt_bin(const CddtDev& c, float theta, bool* fli
 
 // CDDTCast::calc_range (RangeLib.h:1341-1514): the origin is projected onto
 // the bin of the (discretized) heading and the nearest obstacle position
-// ahead of it (behind it when flipped) is searched in that bin: upper_bound
-// above 64 entries, a linear scan below, as the reference.  NaN where the
+// ahead of it (behind it when flipped) is searched in that bin.  The
+// reference uses upper_bound above 64 entries and a linear scan below; the
+// scan's answer is the first entry >= x (lower_bound) or, flipped, the last
+// entry <= x (upper_bound - 1), so both are binary searches here: a few
+// dependent loads instead of up to 64 in sequence, same results.  NaN where the
 // heading's wrap never ends and where the reference tests the occupancy of a
 // cell outside the map (it indexes grid[x][y] out of bounds); 0 where its
 // binary search would read past the bin (the query projects exactly onto the
@@ -394,40 +397,41 @@ __device__ __forceinline__ float cddt_calc_range(const MapDev& m, float max_rang
   if (high == -1) return max_range;
   if (COUNT) *reads = 1;
   const bool inside = x > -1.0f && x < (float)m.width && y > -1.0f && y < (float)m.height;
+  if (lx != lx) {  // a NaN projection (not reached from finite origins): the reference's searches as written
+    if (!inside) return __builtin_nanf("");
+    if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) return 0.0f;
+    if (high > 64) return fl ? lx : 0.0f;
+    return -1.0f;
+  }
   if (fl) {
     if (v[0] > lx) return max_range;
     if (v[high] < lx) return lx - v[high];
     if (!inside) return __builtin_nanf("");
     if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) return 0.0f;
-    if (high > 64) {
-      long long lo = 0, hi = n;  // upper_bound
-      while (lo < hi) {
-        const long long mid = (lo + hi) >> 1;
-        if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
-      }
-      return lx - v[lo - 1];
+    long long lo = 0, hi = n;  // upper_bound (the linear scan's last entry <= x is its predecessor)
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
     }
-    for (long long i = high; i >= 0; --i) {
-      const float o = v[i];
-      if (o <= lx) return lx - o;
-    }
+    return lx - v[lo - 1];  // lo >= 1: v[0] <= lx
   } else {
     if (v[high] < lx) return max_range;
     if (v[0] > lx) return v[0] - lx;
     if (!inside) return __builtin_nanf("");
     if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) return 0.0f;
-    if (high > 64) {
-      long long lo = 0, hi = n;
+    long long lo = 0, hi = n;
+    if (high > 64) {  // upper_bound; past the end only when the last entry equals x: 0, as the scan
       while (lo < hi) {
         const long long mid = (lo + hi) >> 1;
         if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
       }
       return lo == n ? 0.0f : v[lo] - lx;
     }
-    for (long long i = 0; i < n; ++i) {
-      const float o = v[i];
-      if (o >= lx) return o - lx;
+    while (lo < hi) {  // lower_bound: the linear scan's first entry >= x
+      const long long mid = (lo + hi) >> 1;
+      if (v[mid] < lx) lo = mid + 1; else hi = mid;
     }
+    return v[lo] - lx;  // lo < n: v[high] >= lx
   }
   return -1.0f;  // not reached: the checks above bracket lx within the bin
 }
