@@ -115,6 +115,17 @@ int rl_map_prune_cddt(rl_map* map, float max_range);
  * vals[nvals]; bins of angle 0 first.  Any pointer may be NULL. */
 int rl_map_get_cddt(rl_map* map, int* theta_discretization, long long* nbins, long long* nvals, int* widths,
                     float* trans, long long* start, float* vals, float* build_ms);
+/* RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676): num_rays
+ * beams evenly over [min_angle, max_angle] per world-frame pose, the beams one
+ * half-turn apart answered by one CDDT pair query (calc_range_pair,
+ * RangeLib.h:1521-1653; other methods give -1 for those slots, RangeLib.h:419).
+ * Slots the reference leaves unwritten keep their contents; slots it would
+ * write past a particle's num_rays are not written. */
+int rl_calc_range_many_radial_optimized(rl_map* map, int method, float max_range, const float* ins, float* outs,
+                                        int num_particles, int num_rays, float min_angle, float max_angle);
+int rl_calc_range_many_radial_optimized_dev(rl_map* map, int method, float max_range, const float* d_ins,
+                                            float* d_outs, int num_particles, int num_rays, float min_angle,
+                                            float max_angle);
 /* Copies the table (x-major, theta fastest: the reference's giant_lut[x][y][i])
  * when `out` is non-NULL; any other pointer may be NULL. */
 int rl_map_get_giant_lut(rl_map* map, uint16_t* out, int* theta_discretization, float* max_range,

@@ -88,6 +88,7 @@ def _load_oracle():
     lib.or_cddt_build.restype = C.POINTER(_Cddt)
     lib.or_cddt_free.argtypes = [C.POINTER(_Cddt)]
     lib.or_cddt_prune.argtypes = [C.POINTER(_Cddt), _u8p, C.c_int, C.c_int, C.c_float]
+    lib.or_calc_range_many_radial_optimized.argtypes = [mp, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float]
     lib.or_sinf_many.argtypes = [_fp, _fp, C.c_long]
     lib.or_cosf_many.argtypes = [_fp, _fp, C.c_long]
     return lib
@@ -195,6 +196,13 @@ class Oracle:
     def cddt_tables(self) -> dict:
         return cddt_tables(self._cddt)
 
+    def calc_range_many_radial_optimized(self, ins, num_rays, min_angle, max_angle, fill=-7.0):
+        ins = _f32(ins)
+        outs = np.full(ins.shape[0] * num_rays, fill, dtype=np.float32)
+        self.lib.or_calc_range_many_radial_optimized(C.byref(self.m), _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0],
+                                                     num_rays, min_angle, max_angle)
+        return outs
+
     def set_sensor_model(self, table: np.ndarray):
         self.table = np.ascontiguousarray(table, dtype=np.float64)
         assert self.table.shape[0] == self.table.shape[1]
@@ -275,6 +283,8 @@ def ref_lib():
         lib.ref_cddt_prune.argtypes = [C.c_void_p, C.c_float]
         lib.ref_cddt_sizes.argtypes = [C.c_void_p, _llp, _llp]
         lib.ref_cddt_export.argtypes = [C.c_void_p, C.POINTER(C.c_int), _fp, _llp, _fp]
+        lib.ref_calc_range_many_radial_optimized.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_float,
+                                                             C.c_float]
         lib.ref_destroy.argtypes = [C.c_void_p]
         lib.ref_set_sensor_model.argtypes = [C.c_void_p, _dp, C.c_int]
         lib.ref_calc_range.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
@@ -325,6 +335,15 @@ class Reference:
             self.h = self.lib.ref_create_cddt(_ptr(occ, _u8p), w, h, max_range, theta_disc, wp)
         else:
             self.h = self.lib.ref_create(1 if kind == "bl" else 0, _ptr(occ, _u8p), w, h, max_range, wp)
+
+    def calc_range_many_radial_optimized(self, ins, num_rays, min_angle, max_angle, fill=-7.0):
+        """outs has one extra particle's slots of headroom: the reference writes pair
+        results past the last particle's segment for some angle ranges."""
+        ins = _f32(ins)
+        outs = np.full((ins.shape[0] + 1) * num_rays, fill, dtype=np.float32)
+        self.lib.ref_calc_range_many_radial_optimized(self.h, _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0],
+                                                      num_rays, min_angle, max_angle)
+        return outs[: ins.shape[0] * num_rays]
 
     def cddt_prune(self, max_range: float):
         self.lib.ref_cddt_prune(self.h, max_range)

@@ -144,6 +144,11 @@ void ref_cddt_export(void* p, int* widths, float* trans, long long* start, float
   }
 }
 
+// RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676).
+void ref_calc_range_many_radial_optimized(void* p, float* ins, float* outs, int np_, int nr, float mn, float mx) {
+  ((Handle*)p)->method->calc_range_many_radial_optimized(ins, outs, np_, nr, mn, mx);
+}
+
 void ref_destroy(void* p) {
   Handle* hd = (Handle*)p;
   delete hd->method;

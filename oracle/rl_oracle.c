@@ -610,3 +610,105 @@ float or_cddt_calc_range(const or_cddt* c, const uint8_t* occ, int w, int h, flo
   }
   return -1.0f; /* not reached (assert(0) in the reference) */
 }
+
+void or_cddt_calc_range_pair(const or_cddt* c, const uint8_t* occ, int w, int h, float max_range, float x, float y,
+                             float heading, float* r, float* r_inv) {
+  /* CDDTCast::calc_range_pair (RangeLib.h:1521-1653): the range at the heading
+   * and at the heading + pi from one bin search.  The reference indexes its
+   * table without checking the bin (max_range pair here), its unflipped
+   * occupancy test has no effect (no return: followed), and the flipped one
+   * outside the map is NaN here (out of bounds there). */
+  int fl = 0;
+  int a = cddt_discretize(c, (float)(-1.0 * (double)heading), &fl);
+  if (a < 0) {
+    *r = *r_inv = NAN;
+    return;
+  }
+  float cs = c->cosv[a], sn = c->sinv[a];
+  float lx = x * cs - y * sn;
+  float ly = (x * sn + y * cs) + c->trans[a];
+  int li_i = (ly >= -2147483648.0f && ly < 2147483648.0f) ? (int)ly : INT_MIN;
+  unsigned li = (unsigned)li_i;
+  if (li >= (unsigned)c->width[a]) {
+    *r = *r_inv = max_range;
+    return;
+  }
+  long long b = c->bin_off[a] + li, s = c->start[b];
+  long long n = c->start[b + 1] - s;
+  const float* v = c->vals + s;
+  long long high = n - 1;
+  if (high == -1) {
+    *r = *r_inv = max_range;
+    return;
+  }
+  int cx = (x > -1.0f && x < (float)w) ? (int)x : -1, cy = (y > -1.0f && y < (float)h) ? (int)y : -1;
+  if (fl) {
+    if (v[0] > lx) {
+      float d = v[0] - lx;
+      *r = max_range;
+      *r_inv = (d < max_range) ? d : max_range;
+      return;
+    }
+    if (v[high] < lx) {
+      *r = lx - v[high];
+      *r_inv = max_range;
+      return;
+    }
+    if (cx < 0 || cy < 0) {
+      *r = *r_inv = NAN;
+      return;
+    }
+    if (occ[(size_t)cx * h + cy]) {
+      *r = *r_inv = 0.0f;
+      return;
+    }
+    long long index = upper_b(v, n, lx) - 1; /* the linear scan's last entry <= x */
+    float val = v[index];
+    *r = lx - val;
+    *r_inv = (index + 1 == n) ? max_range : v[index + 1] - lx;
+  } else {
+    if (v[high] < lx) {
+      float d = lx - v[high];
+      *r = max_range;
+      *r_inv = (d < max_range) ? d : max_range;
+      return;
+    }
+    long long index = lower_b(v, n, lx); /* the linear scan's first entry >= x */
+    float val = v[index];
+    *r = val - lx;
+    *r_inv = (index - 1 == -1) ? max_range : lx - v[index - 1];
+  }
+}
+
+void or_calc_range_many_radial_optimized(const or_method* m, const float* ins, float* outs, int num_particles,
+                                         int num_rays, float min_angle, float max_angle) {
+  /* RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676).  Pair
+   * queries exist for CDDTCast only (other methods return (-1, -1),
+   * RangeLib.h:419).  Slots at or past num_rays are not written (the reference
+   * writes them into the next particle's, or past the array for the last). */
+  world_consts c = make_consts(&m->world); /* the same constants, RangeLib.h:619-627 */
+  float step = (max_angle - min_angle) / (float)(num_rays - 1);
+  int max_pairwise_index = (int)((double)(float)num_rays / 3.0);
+  float index_offset_float = (float)(((double)num_rays - 1.0) * OR_M_PI / (double)(max_angle - min_angle));
+  int index_offset = (int)roundf(index_offset_float);
+  for (int i = 0; i < num_particles; ++i) {
+    float theta = -ins[3 * i + 2] + c.rotation_const, x, y;
+    to_grid(&c, ins[3 * i], ins[3 * i + 1], &x, &y);
+    float angle = min_angle;
+    for (int a = 0; a <= max_pairwise_index; ++a) {
+      float r, r_inv;
+      if (m->kind == 3) {
+        or_cddt_calc_range_pair(m->cddt, m->occ, m->width, m->height, m->max_range, y, x, theta - angle, &r, &r_inv);
+      } else {
+        r = r_inv = -1.0f;
+      }
+      if (a < num_rays) outs[(size_t)i * num_rays + a] = r * c.world_scale;
+      if (a + index_offset < num_rays) outs[(size_t)i * num_rays + a + index_offset] = r_inv * c.world_scale;
+      angle += step;
+    }
+    for (int a = max_pairwise_index + 1; a < index_offset; ++a) {
+      if (a < num_rays) outs[(size_t)i * num_rays + a] = or_calc_range(m, y, x, theta - angle) * c.world_scale;
+      angle += step;
+    }
+  }
+}

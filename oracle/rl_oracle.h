@@ -106,6 +106,15 @@ void or_cddt_prune(or_cddt* c, const uint8_t* occ, int width, int height, float 
 float or_cddt_calc_range(const or_cddt* c, const uint8_t* occ, int width, int height, float max_range, float x,
                          float y, float heading);
 
+/* CDDTCast::calc_range_pair (RangeLib.h:1521-1653): ranges at heading and heading + pi. */
+void or_cddt_calc_range_pair(const or_cddt* c, const uint8_t* occ, int width, int height, float max_range, float x,
+                             float y, float heading, float* r, float* r_inv);
+
+/* RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676); slots past
+ * num_rays (written out of the particle's segment by the reference) are skipped. */
+void or_calc_range_many_radial_optimized(const or_method* m, const float* ins, float* outs, int num_particles,
+                                         int num_rays, float min_angle, float max_angle);
+
 float or_calc_range(const or_method* m, float x, float y, float heading);
 
 /* RayMarchingGPU::calc_range_many semantics (grid frame, no transform;

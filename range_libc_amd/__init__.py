@@ -264,6 +264,17 @@ class PyCDDTCast(_GpuMethod):
         """RangeLibc.pyx:261-265: a negative max_range prunes at the constructor's."""
         self.map.prune_cddt(self.max_range if max_range < 0.0 else float(np.float32(max_range)))
 
+    def calc_range_many_radial_optimized(self, num_rays, min_angle, max_angle, ins, outs):
+        """RangeLibc.pyx:273-275: num_rays beams over [min_angle, max_angle] per pose,
+        opposite beams answered by one pair query."""
+        _need(ins, np.float32, 2, "ins")
+        _need(outs, np.float32, 1, "outs")
+        n = ins.shape[0]
+        if outs.shape[0] < n * num_rays:
+            raise ValueError(f"outs needs {n * num_rays} elements, has {outs.shape[0]}")
+        self.map.calc_range_many_radial_optimized(self.METHOD, self.max_range, ins, outs, n, int(num_rays),
+                                                  float(min_angle), float(max_angle))
+
     def calc_range(self, x, y, heading):
         return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))
 

@@ -45,6 +45,9 @@ SIGNATURES = {
     "rl_map_destroy": [_vp],
     "rl_map_build_giant_lut": [_vp, C.c_float, C.c_int],
     "rl_map_build_cddt": [_vp, C.c_int],
+    "rl_calc_range_many_radial_optimized": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int, C.c_int, C.c_float, C.c_float],
+    "rl_calc_range_many_radial_optimized_dev": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int, C.c_int, C.c_float,
+                                                C.c_float],
     "rl_map_prune_cddt": [_vp, C.c_float],
     "rl_map_get_cddt": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _vp, _vp, _vp,
                         _vp, _fp],
@@ -253,6 +256,11 @@ class DeviceMap:
             out = np.empty((self.width, self.height, td.value), dtype=np.uint16)
             check(lib().rl_map_get_giant_lut(self.handle, C.c_void_p(out.ctypes.data), None, None, None))
         return out, td.value, r.value, ms.value
+
+    def calc_range_many_radial_optimized(self, method, max_range, ins, outs, num_particles, num_rays, min_angle,
+                                         max_angle):
+        check(lib().rl_calc_range_many_radial_optimized(self.handle, method, max_range, ptr(ins), ptr(outs),
+                                                        num_particles, num_rays, min_angle, max_angle))
 
     def build_cddt(self, theta_disc: int):
         """CDDTCast tables of this map (RL_METHOD_CDDT then works)."""

@@ -163,6 +163,57 @@ __global__ void k_cddt_prune_mark(const uint8_t* __restrict__ occ, int W, int H,
   if (lo >= 1) used[s + lo - 1] = 1;
 }
 
+// RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676), one
+// thread per output slot s of particle p.  The reference loops a = 0..mpi
+// writing pair(a).r to slot a and pair(a).r_inv to slot a + off, then a =
+// mpi+1..off-1 writing calc_range to slot a, the beam angle growing by `step`
+// each iteration; a slot keeps its last write.  So slot s is: the single ray
+// if mpi < s < off; else pair(s - off).r_inv if that iteration exists and
+// either off <= 0 or s > mpi; else pair(s).r if s <= mpi; else untouched.
+// Pairs exist for CDDTCast only (-1, -1 for the other methods, RangeLib.h:419).
+template <int METHOD>
+__global__ __launch_bounds__(256) void k_radial(MapDev m, TrigDev trig, float max_range, const float* __restrict__ ins,
+                                                float* __restrict__ outs, int N, int nr, float min_angle, float step,
+                                                int mpi, int off) {
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (long long)N * nr) return;
+  const int p = (int)(r / nr), s = (int)(r - (long long)p * nr);
+  int kind = 0, a = 0;  // 1: pair r, 2: pair r_inv, 3: single ray
+  if (s > mpi && s < off) {
+    kind = 3;
+    a = s;
+  } else {
+    const bool first = s <= mpi;
+    const int a2 = s - off;
+    const bool second = a2 >= 0 && a2 <= mpi;
+    if (second && (off <= 0 || !first)) {
+      kind = 2;
+      a = a2;
+    } else if (first) {
+      kind = 1;
+      a = s;
+    }
+  }
+  if (kind == 0) return;
+  float angle = min_angle;
+  for (int k = 0; k < a; ++k) angle += step;
+  float gx, gy;
+  rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
+  const float h = (-ins[3 * (size_t)p + 2] + m.world.rotation_const) - angle;
+  float v;
+  if (kind == 3) {
+    int unused;
+    v = cast_one<METHOD, false>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, gy, gx, h, &unused);
+  } else {
+    float r0 = -1.0f, r1 = -1.0f;
+    if (METHOD == RL_METHOD_CDDT) rl::cddt_calc_range_pair(m, max_range, gy, gx, h, &r0, &r1);
+    v = kind == 1 ? r0 : r1;
+  }
+  outs[r] = v * m.world.scale;
+}
+
 namespace {
 
 struct DevBuf {
@@ -339,9 +390,68 @@ int cddt_prune_locked(rl_map* m, float max_range) {
   return cddt_compact(m, 1, used.as<uint8_t>());
 }
 
+template <int METHOD>
+int launch_radial_t(rl_map* m, float max_range, const float* ins, float* outs, int N, int nr, float min_angle,
+                    float step, int mpi, int off) {
+  const long long n = (long long)N * nr;
+  if (n == 0) return RL_OK;
+  hipLaunchKernelGGL(k_radial<METHOD>, dim3(grid_for(n, 256)), dim3(256), 0, m->ctx->stream, m->dev(),
+                     m->ctx->trig, max_range, ins, outs, N, nr, min_angle, step, mpi, off);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+int launch_radial(rl_map* m, int method, float max_range, const float* ins, float* outs, int N, int nr,
+                  float min_angle, float max_angle) {
+  TRY(check_map(m, method, max_range));
+  if ((long long)N * nr > 0x7fffffffLL) return fail(RL_ERR_ARG, "batch too large");
+  // RangeLib.h:631-641
+  const float step = (max_angle - min_angle) / (float)(nr - 1);
+  const int mpi = (int)((double)(float)nr / 3.0);
+  const float iof = (float)(((double)nr - 1.0) * 3.14159265358979323846 / (double)(max_angle - min_angle));
+  const int off = (int)roundf(iof);
+  switch (method) {
+    case RL_METHOD_BL: return launch_radial_t<RL_METHOD_BL>(m, max_range, ins, outs, N, nr, min_angle, step, mpi, off);
+    case RL_METHOD_GLT: return launch_radial_t<RL_METHOD_GLT>(m, max_range, ins, outs, N, nr, min_angle, step, mpi, off);
+    case RL_METHOD_CDDT: return launch_radial_t<RL_METHOD_CDDT>(m, max_range, ins, outs, N, nr, min_angle, step, mpi, off);
+    default: return launch_radial_t<RL_METHOD_RM>(m, max_range, ins, outs, N, nr, min_angle, step, mpi, off);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int rl_calc_range_many_radial_optimized_dev(rl_map* m, int method, float max_range, const float* d_ins,
+                                            float* d_outs, int num_particles, int num_rays, float min_angle,
+                                            float max_angle) {
+  CHECK_ARG(m && d_ins && d_outs && num_particles >= 0 && num_rays >= 2, "bad argument");
+  std::lock_guard<std::mutex> lk(m->ctx->mu);
+  TRY(use_device(m->ctx));
+  return launch_radial(m, method, max_range, d_ins, d_outs, num_particles, num_rays, min_angle, max_angle);
+}
+
+int rl_calc_range_many_radial_optimized(rl_map* m, int method, float max_range, const float* ins, float* outs,
+                                        int num_particles, int num_rays, float min_angle, float max_angle) {
+  CHECK_ARG(m && ins && outs && num_particles >= 0 && num_rays >= 2, "bad argument");
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  const long long n = (long long)num_particles * num_rays;
+  if (n > 0x7fffffffLL) return fail(RL_ERR_ARG, "batch too large");
+  if (n == 0) return RL_OK;
+  void *d_ins, *d_out;
+  TRY(c->staging(0, (size_t)num_particles * 3 * sizeof(float), &d_ins));
+  TRY(c->staging(1, (size_t)n * sizeof(float), &d_out));
+  HIP_TRY(hipMemcpyAsync(d_ins, ins, (size_t)num_particles * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  // slots the reference leaves untouched keep the caller's values
+  HIP_TRY(hipMemcpyAsync(d_out, outs, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  TRY(launch_radial(m, method, max_range, (const float*)d_ins, (float*)d_out, num_particles, num_rays, min_angle,
+                    max_angle));
+  HIP_TRY(hipMemcpyAsync(outs, d_out, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
 
 int rl_map_build_cddt(rl_map* m, int theta_disc) {
   CHECK_ARG(m, "map is NULL");

@@ -436,6 +436,82 @@ __device__ __forceinline__ float cddt_calc_range(const MapDev& m, float max_rang
   return -1.0f;  // not reached: the checks above bracket lx within the bin
 }
 
+// CDDTCast::calc_range_pair (RangeLib.h:1521-1653): the ranges at the
+// heading and at the heading + pi from one bin search.  The reference does not
+// check the bin index (a max_range pair here), its unflipped occupancy test
+// has no effect (no return: followed), and a flipped occupancy test outside
+// the map is NaN here (out of bounds there).  The small-bin linear scans are
+// the same binary searches as in cddt_calc_range.
+__device__ __forceinline__ void cddt_calc_range_pair(const MapDev& m, float max_range, float x, float y,
+                                                     float heading, float* r, float* r_inv) {
+  const CddtDev& c = m.cddt;
+  bool fl;
+  const int a = cddt_bin(c, -heading, &fl);
+  if (a < 0) {
+    *r = *r_inv = __builtin_nanf("");
+    return;
+  }
+  const float cs = c.cosv[a], sn = c.sinv[a];
+  const float lx = x * cs - y * sn;
+  const float ly = (x * sn + y * cs) + c.trans[a];
+  const unsigned li = (unsigned)trunc_x86(ly);
+  if (li >= (unsigned)c.width[a]) {
+    *r = *r_inv = max_range;
+    return;
+  }
+  const long long b = c.bin_off[a] + li, s = c.start[b];
+  const long long n = c.start[b + 1] - s;
+  const RL_GLB float* v = glb(c.vals) + s;
+  const long long high = n - 1;
+  if (high == -1) {
+    *r = *r_inv = max_range;
+    return;
+  }
+  const bool inside = x > -1.0f && x < (float)m.width && y > -1.0f && y < (float)m.height;
+  if (fl) {
+    if (v[0] > lx) {
+      const float d = v[0] - lx;
+      *r = max_range;
+      *r_inv = (d < max_range) ? d : max_range;
+      return;
+    }
+    if (v[high] < lx) {
+      *r = lx - v[high];
+      *r_inv = max_range;
+      return;
+    }
+    if (!inside) {
+      *r = *r_inv = __builtin_nanf("");
+      return;
+    }
+    if (glb(m.occ)[(size_t)(int)x * (unsigned)m.height + (unsigned)(int)y]) {
+      *r = *r_inv = 0.0f;
+      return;
+    }
+    long long lo = 0, hi = n;  // upper_bound: the last entry <= x is lo - 1 (>= 0: v[0] <= x)
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (!(lx < v[mid])) lo = mid + 1; else hi = mid;
+    }
+    *r = lx - v[lo - 1];
+    *r_inv = (lo == n) ? max_range : v[lo] - lx;
+  } else {
+    if (v[high] < lx) {
+      const float d = lx - v[high];
+      *r = max_range;
+      *r_inv = (d < max_range) ? d : max_range;
+      return;
+    }
+    long long lo = 0, hi = n;  // lower_bound: the first entry >= x (< n: v[high] >= x)
+    while (lo < hi) {
+      const long long mid = (lo + hi) >> 1;
+      if (v[mid] < lx) lo = mid + 1; else hi = mid;
+    }
+    *r = v[lo] - lx;
+    *r_inv = (lo == 0) ? max_range : lx - v[lo - 1];
+  }
+}
+
 __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_range) {
   return fabsf(x0) < 1.0e9f && fabsf(y0) < 1.0e9f && max_range < 1.0e9f;
 }

@@ -244,6 +244,9 @@ def cddt():
                 g[f"{name}_{tag}_{k}_sha"] = np.array(sha(t[k]))
             g[f"{name}_{tag}_nvals"] = np.array(len(t["vals"]))
             g[f"{name}_{tag}_grid"] = ref.calc_range_many(q)
+            # calc_range_many_radial_optimized from free-cell world poses, 60 beams over 270 degrees
+            g[f"{name}_{tag}_radial"] = ref.calc_range_many_radial_optimized(
+                W.world_poses(img, 200, seed=6), 60, -3 * math.pi / 4, 3 * math.pi / 4)
         # world-frame queries land outside the map for this world: keep those inside
         refw = O.Reference(img, R, "cddt", world=world, theta_disc=td)
         g[f"{name}_world_rot"] = refw.numpy_calc_range(world_inside(q, world, w, h))

@@ -149,3 +149,40 @@ def test_cddt_special_cases_and_errors(ctx):
     dm.build_cddt(1)  # one bin: rebuilt in place
     t = dm.cddt_tables()
     assert t["widths"].shape == (1,)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_radial_optimized_golden_and_dev(ctx, golden, name):
+    """calc_range_many_radial_optimized: host entry point against the compiled
+    reference's fixture; device entry point against the oracle, slots the
+    reference leaves unwritten keeping their contents; other methods give -1 pairs."""
+    torch = _torch()
+    R, td = CASES[name]
+    img = golden[f"{name}_img"]
+    ins = W.world_poses(img, 200, seed=6)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    dm.build_cddt(td)
+    for tag in ("full", "pruned"):
+        if tag == "pruned":
+            dm.prune_cddt(R)
+        out = np.full(200 * 60, -7.0, np.float32)
+        dm.calc_range_many_radial_optimized(rl.RL_METHOD_CDDT, R, ins, out, 200, 60, -3 * math.pi / 4,
+                                            3 * math.pi / 4)
+        assert_bits_equal(out, golden[f"{name}_{tag}_radial"], f"{tag} radial")
+    o = O.Oracle(img, R, "cddt", theta_disc=td, prune=True)
+    ins2 = W.world_poses(img, 90, seed=9)
+    for nr, mn, mx in ((100, -math.pi, math.pi), (31, -math.pi / 2, math.pi / 2 + 0.3), (9, -4.0, 7.0)):
+        d_ins = torch.from_numpy(ins2).cuda()
+        d_out = torch.full((90 * nr,), -7.0, dtype=torch.float32, device="cuda")
+        lib = rl._lib.lib()
+        rl._lib.check(lib.rl_calc_range_many_radial_optimized_dev(dm.handle, rl.RL_METHOD_CDDT, R,
+                                                                  rl._lib._vp(d_ins.data_ptr()),
+                                                                  rl._lib._vp(d_out.data_ptr()), 90, nr, mn, mx))
+        ctx.sync()
+        assert_bits_equal(d_out.cpu().numpy(), o.calc_range_many_radial_optimized(ins2, nr, mn, mx, fill=-7.0),
+                          f"dev {nr}")
+    rm = rl.DeviceMap(ctx, O.xmajor(img))
+    out = np.full(50 * 60, -7.0, np.float32)
+    rm.calc_range_many_radial_optimized(rl.RL_METHOD_RM, R, ins[:50], out, 50, 60, -3 * math.pi / 4, 3 * math.pi / 4)
+    assert_bits_equal(out, O.Oracle(img, R, "rm").calc_range_many_radial_optimized(ins[:50], 60, -3 * math.pi / 4,
+                                                                                    3 * math.pi / 4), "rm pairs")

@@ -463,3 +463,30 @@ def test_cddt_oracle_matches_compiled_reference(seed):
         q = np.stack([rng.uniform(0, w, 3000), rng.uniform(0, h, 3000), rng.uniform(-100, 100, 3000)],
                      1).astype(np.float32)
         assert_bits_equal(o.calc_range_many(q), r.calc_range_many(q), "grid")
+
+
+@pytest.mark.parametrize("name", sorted(CDDT_CASES))
+def test_cddt_radial_optimized_golden(name):
+    g = np.load(os.path.join(GOLDEN, "cddt.npz"))
+    R, td = CDDT_CASES[name]
+    img = g[f"{name}_img"]
+    ins = W.world_poses(img, 200, seed=6)
+    for tag, prune in (("full", False), ("pruned", True)):
+        o = O.Oracle(img, R, "cddt", theta_disc=td, prune=prune)
+        got = o.calc_range_many_radial_optimized(ins, 60, -3 * math.pi / 4, 3 * math.pi / 4)
+        assert_bits_equal(got, g[f"{name}_{tag}_radial"], f"{name} {tag} radial")
+
+
+@needs_ref
+@pytest.mark.parametrize("nr,mn,mx", [(60, -3 * math.pi / 4, 3 * math.pi / 4), (100, -math.pi, math.pi),
+                                      (31, -math.pi / 2, math.pi / 2 + 0.3)])
+def test_radial_optimized_matches_compiled_reference(nr, mn, mx):
+    img = W.corridor_map(200, seed=3)
+    ins = W.world_poses(img, 150, seed=nr)
+    for kind, td, prune in (("cddt", 108, False), ("cddt", 37, True), ("rm", 0, False)):
+        ref = O.Reference(img, 150.0, kind, theta_disc=td)
+        if prune:
+            ref.cddt_prune(150.0)
+        o = O.Oracle(img, 150.0, kind, theta_disc=td, prune=prune)
+        assert_bits_equal(o.calc_range_many_radial_optimized(ins, nr, mn, mx),
+                          ref.calc_range_many_radial_optimized(ins, nr, mn, mx), f"{kind} {td} {prune}")
