@@ -285,6 +285,8 @@ def ref_lib():
         lib.ref_cddt_export.argtypes = [C.c_void_p, C.POINTER(C.c_int), _fp, _llp, _fp]
         lib.ref_calc_range_many_radial_optimized.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_float,
                                                              C.c_float]
+        lib.ref_cddt_serialize_json.argtypes = [C.c_void_p, C.c_char_p, C.c_longlong]
+        lib.ref_cddt_serialize_json.restype = C.c_longlong
         lib.ref_destroy.argtypes = [C.c_void_p]
         lib.ref_set_sensor_model.argtypes = [C.c_void_p, _dp, C.c_int]
         lib.ref_calc_range.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float]
@@ -344,6 +346,12 @@ class Reference:
         self.lib.ref_calc_range_many_radial_optimized(self.h, _ptr(ins, _fp), _ptr(outs, _fp), ins.shape[0],
                                                       num_rays, min_angle, max_angle)
         return outs[: ins.shape[0] * num_rays]
+
+    def cddt_json(self) -> str:
+        n = self.lib.ref_cddt_serialize_json(self.h, None, 0)
+        buf = C.create_string_buffer(n + 1)
+        self.lib.ref_cddt_serialize_json(self.h, buf, n + 1)
+        return buf.raw[:n].decode()
 
     def cddt_prune(self, max_range: float):
         self.lib.ref_cddt_prune(self.h, max_range)

@@ -149,6 +149,19 @@ void ref_calc_range_many_radial_optimized(void* p, float* ins, float* outs, int 
   ((Handle*)p)->method->calc_range_many_radial_optimized(ins, outs, np_, nr, mn, mx);
 }
 
+// CDDTCast::serializeJson (RangeLib.h:1690-1735) into out (cap bytes); returns the length.
+long long ref_cddt_serialize_json(void* p, char* out, long long cap) {
+  std::stringstream ss;
+  ((ranges::CDDTCast*)((Handle*)p)->method)->serializeJson(&ss);
+  std::string str = ss.str();
+  if (out && cap > 0) {
+    long long n = (long long)str.size() < cap - 1 ? (long long)str.size() : cap - 1;
+    memcpy(out, str.data(), (size_t)n);
+    out[n] = 0;
+  }
+  return (long long)str.size();
+}
+
 void ref_destroy(void* p) {
   Handle* hd = (Handle*)p;
   delete hd->method;

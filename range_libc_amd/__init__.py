@@ -25,7 +25,7 @@ from ._lib import (RL_FRAME_ANGLES, RL_FRAME_GRID, RL_FRAME_WORLD, RL_METHOD_BL,
 __all__ = [
     "PyOMap", "PyRayMarchingGPU", "PyRayMarching", "PyBresenhamsLine", "PyCDDTCast", "PyGiantLUTCast",
     "PyNull",
-    "RangeLibError", "Context", "DeviceMap", "Batch", "Surrogate", "default_context",
+    "RangeLibError", "Context", "DeviceMap", "Batch", "Surrogate", "default_context", "cddt_json",
     "USE_CACHED_TRIG", "USE_ALTERNATE_MOD", "USE_CACHED_CONSTANTS", "USE_FAST_ROUND", "NO_INLINE",
     "USE_LRU_CACHE", "LRU_CACHE_SIZE", "SHOULD_USE_CUDA",
 ]
@@ -69,6 +69,7 @@ class PyOMap:
 
     def __init__(self, arg1, arg2=None):
         self._error = False
+        self.fn = ""  # OMap::fn: the PNG path, empty otherwise (RangeLib.h:143, 159)
         self.world = (1.0, 0.0, 0.0, 0.0, 0.0, 1.0)  # RangeLibc.pyx:174-180
         if arg1 is not None and arg2 is not None:
             if isinstance(arg1, (int, np.integer)) and isinstance(arg2, (int, np.integer)):
@@ -89,6 +90,7 @@ class PyOMap:
             self.occ = np.zeros((1, 1), dtype=np.uint8)
 
     def _load_png(self, path: str, threshold: float):
+        self.fn = path
         try:
             gray = _rgb2gray_png(path)
         except Exception as e:  # OMap prints the decoder error and flags has_error
@@ -161,6 +163,7 @@ class _GpuMethod:
         if not isinstance(Map, PyOMap):
             raise TypeError("Map must be a PyOMap")
         self.max_range = float(np.float32(max_range))
+        self.omap = Map
         self.map = DeviceMap(default_context(device), Map.occ)
         self.map.set_world(*Map.world)
 
@@ -249,6 +252,43 @@ class PyBresenhamsLine(_GpuMethod):
         return self.map.calc_range(self.METHOD, self.max_range, float(x), float(y), float(heading))
 
 
+def _g7(v) -> str:
+    """A float as std::ostream prints it at setprecision(7): C's %.7g of the value."""
+    return "%.7g" % float(np.float32(v))
+
+
+def _vec(vals) -> str:
+    """utils::serialize(std::vector<float>) (RangeUtils.h:222-232)."""
+    return "[" + ",".join(_g7(v) for v in vals) + "]"
+
+
+def cddt_json(theta_disc, trans, max_range, fn, occ_xmajor, widths, start, vals) -> str:
+    """CDDTCast::serializeJson (RangeLib.h:1690-1735) from CSR tables: the same text,
+    line by line (J1 = two spaces; map data x-major as grid[x]; theta i * (2 pi / td))."""
+    td = int(theta_disc)
+    occ = np.asarray(occ_xmajor)
+    w, h = occ.shape
+    step = np.float32(6.28318530718 / np.float64(np.float32(td)))
+    out = ['{"cddt": {\n', '  "theta_discretization": %d,\n' % td, '  "lut_translations": ' + _vec(trans) + ",\n",
+           '  "max_range":' + _g7(max_range) + ",\n", '  "map": {\n', '    "path": "' + fn + '",\n',
+           '    "width": %d,\n' % w, '    "height": %d,\n' % h,
+           '    "data": [' + ",".join("[" + ",".join("1" if c else "0" for c in occ[x]) + "]" for x in range(w)) + "],\n",
+           "  },\n", '  "compressed_lut": [\n']
+    b = 0
+    for i in range(td):
+        out.append("    {\n")
+        out.append('      "theta": ' + _g7(np.float32(i) * step) + ",\n")
+        bins = []
+        for _ in range(int(widths[i])):
+            bins.append(_vec(vals[start[b]:start[b + 1]]))
+            b += 1
+        out.append('      "zeros": [' + ",".join(bins) + "]\n")
+        out.append("    },\n" if i < td - 1 else "    }\n")
+    out.append("  ]\n")
+    out.append("}}\n")
+    return "".join(out)
+
+
 class PyCDDTCast(_GpuMethod):
     """CDDTCast (RangeLib.h:971-1770; RangeLibc.pyx:253-287): compressed directional
     distance transform, tables built on the GPU at construction; prune() makes it PCDDT."""
@@ -263,6 +303,12 @@ class PyCDDTCast(_GpuMethod):
     def prune(self, max_range: float = -1.0):
         """RangeLibc.pyx:261-265: a negative max_range prunes at the constructor's."""
         self.map.prune_cddt(self.max_range if max_range < 0.0 else float(np.float32(max_range)))
+
+    def serializeJson(self) -> str:
+        """CDDTCast::serializeJson (RangeLib.h:1690-1735), the text the reference writes."""
+        t = self.map.cddt_tables()
+        return cddt_json(self.theta_disc, t["trans"], self.max_range, self.omap.fn, self.omap.occ, t["widths"],
+                         t["start"], t["vals"])
 
     def calc_range_many_radial_optimized(self, num_rays, min_angle, max_angle, ins, outs):
         """RangeLibc.pyx:273-275: num_rays beams over [min_angle, max_angle] per pose,

@@ -243,6 +243,7 @@ def cddt():
             for k in ("widths", "trans", "start", "vals"):
                 g[f"{name}_{tag}_{k}_sha"] = np.array(sha(t[k]))
             g[f"{name}_{tag}_nvals"] = np.array(len(t["vals"]))
+            g[f"{name}_{tag}_json_sha"] = np.array(hashlib.sha256(ref.cddt_json().encode()).hexdigest())
             g[f"{name}_{tag}_grid"] = ref.calc_range_many(q)
             # calc_range_many_radial_optimized from free-cell world poses, 60 beams over 270 degrees
             g[f"{name}_{tag}_radial"] = ref.calc_range_many_radial_optimized(

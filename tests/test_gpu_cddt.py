@@ -52,6 +52,8 @@ def test_cddt_tables_and_queries_golden(ctx, golden, name):
         assert _sha(t["trans"]) == str(golden[f"{name}_{tag}_trans_sha"])
         assert _sha(t["start"]) == str(golden[f"{name}_{tag}_start_sha"])
         assert _sha(t["vals"]) == str(golden[f"{name}_{tag}_vals_sha"]), f"{tag}: {len(t['vals'])} values"
+        text = rl.cddt_json(td, t["trans"], R, "", O.xmajor(img), t["widths"], t["start"], t["vals"])
+        assert hashlib.sha256(text.encode()).hexdigest() == str(golden[f"{name}_{tag}_json_sha"])
         out = np.empty(q.shape[0], np.float32)
         dm.calc_range_many(rl.RL_METHOD_CDDT, R, q, out, q.shape[0])
         assert_bits_equal(out, golden[f"{name}_{tag}_grid"], f"{tag} grid")
@@ -123,6 +125,9 @@ def test_py_cddt_cast(ctx):
     assert_bits_equal(out, o.numpy_calc_range(ins), "full")
     c.prune()
     op = O.Oracle(img, 80.0, "cddt", theta_disc=72, prune=True)
+    tp = op.cddt_tables()
+    assert c.serializeJson() == rl.cddt_json(72, tp["trans"], 80.0, "", O.xmajor(img), tp["widths"], tp["start"],
+                                             tp["vals"])
     c.calc_range_many(ins, out)
     assert_bits_equal(out, op.numpy_calc_range(ins), "pruned")
     assert c.calc_range(20.5, 30.25, 0.7) == op.calc_range(20.5, 30.25, 0.7)

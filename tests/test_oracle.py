@@ -490,3 +490,17 @@ def test_radial_optimized_matches_compiled_reference(nr, mn, mx):
         o = O.Oracle(img, 150.0, kind, theta_disc=td, prune=prune)
         assert_bits_equal(o.calc_range_many_radial_optimized(ins, nr, mn, mx),
                           ref.calc_range_many_radial_optimized(ins, nr, mn, mx), f"{kind} {td} {prune}")
+
+
+@pytest.mark.parametrize("name", sorted(CDDT_CASES))
+def test_cddt_json_golden(name):
+    """cddt_json over the oracle's tables reproduces the reference's serializeJson text."""
+    import range_libc_amd as rl
+
+    g = np.load(os.path.join(GOLDEN, "cddt.npz"))
+    R, td = CDDT_CASES[name]
+    img = g[f"{name}_img"]
+    for tag, prune in (("full", False), ("pruned", True)):
+        t = O.Oracle(img, R, "cddt", theta_disc=td, prune=prune).cddt_tables()
+        text = rl.cddt_json(td, t["trans"], R, "", O.xmajor(img), t["widths"], t["start"], t["vals"])
+        assert hashlib.sha256(text.encode()).hexdigest() == str(g[f"{name}_{tag}_json_sha"])
