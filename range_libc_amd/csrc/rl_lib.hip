@@ -6,6 +6,8 @@
 //   k_edt_init      f = 0 (occupied) / FLT_MAX (free) for the envelope pass
 //   k_edt_envelope  distance-transform lower envelope, one line per lane,
 //                   replicating vendor/distance_transform.h:1054-1096
+//   k_edt_rows      pass 2 as an exact bounded search in LDS on the rows where
+//                   it gives the envelope's bits; the others go to k_edt_envelope
 //   k_cast          one ray per lane: ray marching or Bresenham, grid / world /
 //                   repeated-angles frames
 //   k_sensor        eval_sensor_model: one particle per wave segment
@@ -106,9 +108,11 @@ __global__ __launch_bounds__(64) void k_edt_envelope(const float* __restrict__ f
                                                      float* __restrict__ D, int n_lines, int n,
                                                      long long es, long long ls,
                                                      int* __restrict__ vbuf,
-                                                     double* __restrict__ zbuf, int do_sqrt) {
+                                                     double* __restrict__ zbuf, int do_sqrt,
+                                                     const int* __restrict__ only) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= n_lines) return;
+  if (only && !only[l]) return;
   const float* fl = f + (long long)l * ls;
   float* Dl = D + (long long)l * ls;
   int* v = vbuf + l;
@@ -145,6 +149,85 @@ __global__ __launch_bounds__(64) void k_edt_envelope(const float* __restrict__ f
     float dq = (float)q - (float)vk;
     float val = fl[(long long)vk * es] + dq * dq;
     Dl[(long long)q * es] = do_sqrt ? __builtin_sqrtf(val) : val;
+  }
+}
+
+// Pass 2 along x for EDT_RB consecutive rows per workgroup, as a bounded
+// search in LDS instead of the sequential lower envelope, on rows where the two
+// provably give the same bits.
+//
+// Why they agree: let every finite input of a row be a non-negative integer
+// with g[x'] + x'^2 < 2^24, and W <= 4096.  Then the envelope's float sums
+// f[q] + q^2 are exact, every intersection it forms is a correctly rounded
+// quotient of two exact integers with a denominator of at most 2W, and two
+// distinct such quotients differ by far more than their rounding, so each
+// comparison it makes has the exact outcome: the parabola it assigns to q is
+// an exact argmin of g[x'] + (q - x')^2.  Its output float(g[v] + (q - v)^2)
+// is therefore the minimum over x' of the same float expression (rounding is
+// monotone and (q - x')^2 is exact below 4096).  FLT_MAX inputs (columns
+// without obstacles) only ever intersect the finite ones at magnitudes beyond
+// 1e34, so they are never assigned while a finite input exists, and give
+// FLT_MAX when none does -- as the minimum does.
+//
+// The search from x stops at the first distance d with d^2 >= best: every
+// candidate beyond it is at least d^2.  Rows whose inputs break the condition
+// are marked in slow[y] (every row's mark is written) and left to
+// k_edt_envelope.  LDS holds the rows x-major, s[x * EDT_RB + r], the order
+// they sit in global memory, so loads, searches and stores are unit-stride
+// across the lanes.
+constexpr int EDT_RB = 4;
+__global__ __launch_bounds__(256) void k_edt_rows(const float* __restrict__ g,
+                                                  float* __restrict__ D, int W, int H,
+                                                  int* __restrict__ slow, int do_sqrt) {
+  extern __shared__ float s_g[];
+  __shared__ int s_bad[EDT_RB], s_fin[EDT_RB];
+  const int y0 = blockIdx.x * EDT_RB;
+  const int t = threadIdx.x;
+  const int n = EDT_RB * W;
+  if (t < EDT_RB) {
+    s_bad[t] = 0;
+    s_fin[t] = 0;
+  }
+  __syncthreads();
+  for (int c = t; c < n; c += 256) {
+    const int r = c % EDT_RB, x = c / EDT_RB, y = y0 + r;
+    float v = FLT_MAX;
+    if (y < H) {
+      v = rl::glb(g)[(size_t)x * H + y];
+      if (v != FLT_MAX) {
+        const bool exact = v >= 0.0f && v == floorf(v) &&
+                           (double)v + (double)x * (double)x < 16777216.0;
+        if (exact)
+          s_fin[r] = 1;
+        else
+          s_bad[r] = 1;
+      }
+    }
+    s_g[c] = v;
+  }
+  __syncthreads();
+  if (t < EDT_RB && y0 + t < H) slow[y0 + t] = s_bad[t];
+  for (int c = t; c < n; c += 256) {
+    const int r = c % EDT_RB, x = c / EDT_RB, y = y0 + r;
+    if (y >= H || s_bad[r]) continue;
+    float best = FLT_MAX;
+    if (s_fin[r]) {
+      best = s_g[c];
+      const int dmax = max(x, W - 1 - x);
+      for (int d = 1; d <= dmax; ++d) {
+        const float dd = (float)(d * d);
+        if (dd >= best) break;
+        if (x >= d) {
+          const float v = s_g[c - d * EDT_RB] + dd;
+          best = v < best ? v : best;
+        }
+        if (x + d < W) {
+          const float v = s_g[c + d * EDT_RB] + dd;
+          best = v < best ? v : best;
+        }
+      }
+    }
+    D[(size_t)x * H + y] = do_sqrt ? __builtin_sqrtf(best) : best;
   }
 }
 
@@ -908,7 +991,7 @@ int build_edt(rl_map* m, unsigned flags) {
   int rc = RL_OK;
   do {
     if (hipMalloc(&g, N * sizeof(float)) != hipSuccess ||
-        hipMalloc(&v, N * sizeof(int)) != hipSuccess ||
+        hipMalloc(&v, (N + (size_t)H) * sizeof(int)) != hipSuccess ||
         hipMalloc(&z, (N + (size_t)W + (size_t)H) * sizeof(double)) != hipSuccess) {
       (void)hipGetLastError();
       rc = fail(RL_ERR_NOMEM, "device allocation for the %dx%d distance transform failed", W, H);
@@ -924,10 +1007,21 @@ int build_edt(rl_map* m, unsigned flags) {
       hipLaunchKernelGGL(k_edt_init, dim3(grid_for((long long)N, 256)), dim3(256), 0, c->stream,
                          m->d_occ, m->d_edt, N);
       hipLaunchKernelGGL(k_edt_envelope, dim3(grid_for(W, 64)), dim3(64), 0, c->stream,
-                         (const float*)m->d_edt, g, W, H, 1LL, (long long)H, v, z, 0);
+                         (const float*)m->d_edt, g, W, H, 1LL, (long long)H, v, z, 0,
+                         (const int*)nullptr);
+    }
+    // Pass 2: the bounded search where it is exact, the lower envelope on the
+    // rows it marks (or on every row when the envelope is asked for).
+    const int* rows = nullptr;
+    if (W <= 4096 && !(flags & RL_MAP_EDT_ENVELOPE_ONLY)) {
+      int* slow = v + N;
+      hipLaunchKernelGGL(k_edt_rows, dim3((H + EDT_RB - 1) / EDT_RB), dim3(256),
+                         (size_t)EDT_RB * W * sizeof(float), c->stream, (const float*)g,
+                         m->d_edt, W, H, slow, 1);
+      rows = slow;
     }
     hipLaunchKernelGGL(k_edt_envelope, dim3(grid_for(H, 64)), dim3(64), 0, c->stream,
-                       (const float*)g, m->d_edt, H, W, (long long)H, 1LL, v, z, 1);
+                       (const float*)g, m->d_edt, H, W, (long long)H, 1LL, v, z, 1, rows);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

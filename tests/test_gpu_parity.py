@@ -68,13 +68,21 @@ def test_sqrtf_correctly_rounded(ctx):
 
 # --------------------------------------------------------------------------- distance transform
 
+def _points(shape, pts):
+    img = np.zeros(shape, np.uint8)
+    for y, x in pts:
+        img[y, x] = 1
+    return img
+
+
 def _edt_gpu(ctx, img, flags=0):
     m = rl.DeviceMap(ctx, O.xmajor(img), flags)
     return m.edt()
 
 
 @pytest.mark.parametrize("case", ["rect512", "corridor1024", "empty", "full", "single_free",
-                                  "one_px", "tall", "wide", "sparse_cols", "very_tall"])
+                                  "one_px", "tall", "wide", "sparse_cols", "very_tall", "mixed_rows",
+                                  "wider_than_4096"])
 def test_edt_bit_exact(ctx, case):
     rng = np.random.default_rng(7)
     img = {
@@ -88,6 +96,9 @@ def test_edt_bit_exact(ctx, case):
         "wide": lambda: (rng.random((9, 800)) < 0.01).astype(np.uint8),
         "sparse_cols": lambda: (rng.random((300, 257)) < 0.0005).astype(np.uint8),
         "very_tall": lambda: (rng.random((4500, 6)) < 0.003).astype(np.uint8),  # > 4096 rows
+        # rows y >= 926 have g + x^2 >= 2^24 at x = 3990: the row search hands them to the envelope
+        "mixed_rows": lambda: _points((1500, 4000), [(0, 3990), (700, 10)]),
+        "wider_than_4096": lambda: (rng.random((5, 4200)) < 0.002).astype(np.uint8),  # envelope on every row
     }[case]()
     want = O.edt(img)
     assert_bits_equal(_edt_gpu(ctx, img), want, f"edt {case}")
