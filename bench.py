@@ -546,6 +546,31 @@ def main():
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     step_ms = graph_ms
 
+    # (4) the host-buffer API (pageable numpy in and out, as the reference's
+    # Python wrapper is called): the PCIe-inclusive rate, never `value`
+    h_rng = np.empty(rays, np.float32)
+    h_w = np.empty(PARTICLES, np.float64)
+    h_wf = np.empty(PARTICLES, np.float64)
+
+    def host_two_calls():
+        m.numpy_calc_range_angles(rl.RL_METHOD_RM, MAX_RANGE, ins, angles, h_rng, PARTICLES, BEAMS)
+        m.eval_sensor_model(obs, h_rng, h_w, BEAMS, PARTICLES)
+
+    def host_fused():
+        m.repeat_angles_eval_sensor_model(rl.RL_METHOD_RM, MAX_RANGE, ins, angles, obs, h_wf, PARTICLES, BEAMS)
+
+    host = {}
+    for name, fn in (("numpy_calc_range_angles+eval_sensor_model", host_two_calls),
+                     ("calc_range_repeat_angles_eval_sensor_model", host_fused)):
+        for _ in range(5):
+            fn()
+        reps = max(20, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        host[name] = (time.perf_counter() - t0) * 1e3 / reps
+    assert np.array_equal(h_w, w_eager) and np.array_equal(h_rng, r_eager), "host API differs from the device step"
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_rm_angles_pf_step.json")
     if os.path.exists(tpath):
@@ -577,6 +602,8 @@ def main():
         "two_launch_eager_ms_per_step": two_launch_ms,
         "graph_group": GROUP,
         "edt_build_ms": edt_ms,
+        "host_api_ms_per_step": {k: round(v, 4) for k, v in host.items()},
+        "host_api_rays_per_s": {k: rays / (v * 1e-3) for k, v in host.items()},
         "wall_s_timed": wall,
     }
     if not args.no_cpu:
