@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
   __shared__ float lut[RL_LUT_SIZE];
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
   if (METHOD == RL_METHOD_RM) {  // one distance table per context: the same for every map
-    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = maps[0].m.lut[i];
+    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = maps[0].m.steps[i];
     __syncthreads();
   }
   const int lane = threadIdx.x & 63;

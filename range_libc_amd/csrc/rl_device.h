@@ -69,7 +69,7 @@ struct MapDev {
   const float* edt;     // x-major W*H
   const uint8_t* occ;   // x-major W*H
   const uint8_t* idx8;  // blocked index map (see above)
-  const float* lut;     // RL_LUT_SIZE distances, lut[RL_IDX8_ESCAPE] unused
+  const float* steps;   // RL_LUT_SIZE march steps max(d * 0.999f, 1) of the tabulated distances
   int width, height;
   int nby;              // blocks along y: (height + 7) / 8
   int use_box;          // stage the 64x64 cells around a particle in LDS
@@ -182,15 +182,24 @@ __device__ __forceinline__ unsigned idx8_code(const MapDev& m, const Box& box, u
     unsigned ib = (unsigned)((int)(ux >> 3) - box.bx0), jb = (unsigned)((int)(uy >> 3) - box.by0);
     if (ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS) return lds(box.lds)[((ib * RL_BOX_BLOCKS + jb) << 6) | inner];
   }
-  return glb(m.idx8)[((size_t)((ux >> 3) * (unsigned)m.nby + (uy >> 3)) << 6) | inner];
+  // 32-bit offset from the map's base (the compact map is built only below
+  // 4 GiB): one multiply-add and a scalar-base load on the march's chain
+  return glb(m.idx8)[((__umul24(ux >> 3, (unsigned)m.nby) + (uy >> 3)) << 6) | inner];
 }
 
-// Distance of a cell with non-zero code c.
-__device__ __forceinline__ float idx8_dist(const MapDev& m, const float* lut, unsigned c, int px, int py) {
-  return (c != RL_IDX8_ESCAPE) ? lds(lut)[c] : glb(m.edt)[(size_t)px * (unsigned)m.height + py];
+// March step max(d * 0.999f, 1.0f) from a cell with non-zero code c: from the
+// step table staged in LDS (`lut`, the same float operations done once per
+// code), or from the fp32 map for the escape code.
+__device__ __forceinline__ float idx8_step(const MapDev& m, const float* lut, unsigned c, int px, int py) {
+  float g = lds(lut)[c];  // lut[RL_IDX8_ESCAPE] exists: the read needs no branch
+  if (c == RL_IDX8_ESCAPE) {
+    const float st = glb(m.edt)[(size_t)px * (unsigned)m.height + py] * 0.999f;
+    g = (st < 1.0f) ? 1.0f : st;
+  }
+  return g;
 }
 
-// The same march over the compact map.  `lut` is the distance table staged in
+// The same march over the compact map.  `lut` is the step table staged in
 // LDS.  Valid when |x0|, |y0| and max_range are below 1e9 (checked by the
 // caller): every x0 + dx*t then lies in int range, so the conversion is a
 // plain truncation.  Identical results to rm_march: a cell's code is 0 exactly
@@ -208,13 +217,12 @@ __device__ __forceinline__ float idx8_dist(const MapDev& m, const float* lut, un
 template <bool COUNT, bool BOX>
 __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* lut, const Box& box,
                                                  float max_range, float x0, float y0, float dx,
-                                                 float dy, int* reads) {
+                                                 float dy, int* reads, float t, float g,
+                                                 unsigned gcode, int nr) {
+  // (t, g, gcode, nr): position, last step, the code that gave it (256 matches
+  // no cell) and reads so far; (0, 1, 1, 0) for a march from its origin
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
-  float t = 0.0f;
-  float g = 1.0f;       // last step (code 1 is distance 1: step 1)
-  unsigned gcode = 1u;  // the code that gave step g; 256 matches no cell
-  int nr = 0;
-  for (int i = 0; (RL_SPEC == 1 || i < RL_SPEC_AFTER) && t < max_range; ++i) {
+  for (int i = nr; (RL_SPEC == 1 || i < RL_SPEC_AFTER) && t < max_range; ++i) {
     int px = (int)(x0 + dx * t);
     int py = (int)(y0 + dy * t);
     if ((unsigned)px >= W || (unsigned)py >= H) {
@@ -229,8 +237,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
       if (COUNT) *reads = nr;
       return __builtin_sqrtf(xd * xd + yd * yd);
     }
-    float st = idx8_dist(m, lut, c, px, py) * 0.999f;
-    g = (st < 1.0f) ? 1.0f : st;
+    g = idx8_step(m, lut, c, px, py);
     gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
     t += g;
   }
@@ -266,8 +273,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
           return __builtin_sqrtf(xd * xd + yd * yd);
         }
         if (c != gcode || k == RL_SPEC - 1) {
-          float st = idx8_dist(m, lut, c, px[k], py[k]) * 0.999f;
-          g = (st < 1.0f) ? 1.0f : st;
+          g = idx8_step(m, lut, c, px[k], py[k]);
           gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
           t = tk[k] + g;
           break;
@@ -288,8 +294,49 @@ template <bool COUNT>
 __device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut, const Box& box,
                                                float max_range, float x0, float y0, float dx,
                                                float dy, int* reads) {
-  if (box.lds) return rm_march_idx8_t<COUNT, true>(m, lut, box, max_range, x0, y0, dx, dy, reads);
-  return rm_march_idx8_t<COUNT, false>(m, lut, box, max_range, x0, y0, dx, dy, reads);
+  if (dx != dx || dy != dy) {  // see rm_cast_idx8
+    if (COUNT) *reads = 0;
+    return max_range;
+  }
+  if (box.lds) return rm_march_idx8_t<COUNT, true>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
+  return rm_march_idx8_t<COUNT, false>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
+}
+
+// RayMarching::calc_range (RangeLib.h:927-962) over the compact map, from the
+// heading.  The first read is issued before the direction is known: at t = 0
+// the position x0 + dx * 0 is x0 for every finite dx, so the origin cell's
+// load overlaps the sinf/cosf evaluation.  A NaN direction (NaN or infinite
+// heading) returns max_range, as in the reference, whose float-to-int
+// conversion of a NaN position gives INT_MIN, a cell outside the map.
+template <bool COUNT, bool BOX>
+__device__ __forceinline__ float rm_cast_idx8_t(const MapDev& m, const float* lut, const Box& box,
+                                                const TrigDev& trig, float max_range, float x0, float y0,
+                                                float heading, int* reads) {
+  const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
+  const int px = (int)x0, py = (int)y0;
+  const bool in = (unsigned)px < W && (unsigned)py < H;
+  const unsigned c = in ? idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py) : 0u;
+  float dy, dx;
+  sincos_libc(heading, trig, &dy, &dx);
+  if (COUNT) *reads = 0;
+  if (!(0.0f < max_range) || dx != dx || dy != dy || !in) return max_range;
+  if (COUNT) *reads = 1;
+  if (c == 0u) {
+    const float xd = (float)px - x0;
+    const float yd = (float)py - y0;
+    return __builtin_sqrtf(xd * xd + yd * yd);
+  }
+  const float g = idx8_step(m, lut, c, px, py);
+  return rm_march_idx8_t<COUNT, BOX>(m, lut, box, max_range, x0, y0, dx, dy, reads, g, g,
+                                     (c != RL_IDX8_ESCAPE) ? c : 256u, 1);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ float rm_cast_idx8(const MapDev& m, const float* lut, const Box& box,
+                                              const TrigDev& trig, float max_range, float x0, float y0,
+                                              float heading, int* reads) {
+  if (box.lds) return rm_cast_idx8_t<COUNT, true>(m, lut, box, trig, max_range, x0, y0, heading, reads);
+  return rm_cast_idx8_t<COUNT, false>(m, lut, box, trig, max_range, x0, y0, heading, reads);
 }
 
 // GiantLUTCast::discretize_theta (RangeLib.h:1832-1864) with

@@ -265,11 +265,11 @@ __device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, con
                                           float heading, int* cnt) {
   if (METHOD == RL_METHOD_GLT) return rl::glt_calc_range<COUNT>(m, x, y, heading, cnt);
   if (METHOD == RL_METHOD_CDDT) return rl::cddt_calc_range<COUNT>(m, max_range, x, y, heading, cnt);
+  if (METHOD == RL_METHOD_RM && m.idx8 && rl::fast_march_ok(x, y, max_range))
+    return rl::rm_cast_idx8<COUNT>(m, lut, box, trig, max_range, x, y, heading, cnt);
   float s, c;
   rl::sincos_libc(heading, trig, &s, &c);
   if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
-  if (m.idx8 && rl::fast_march_ok(x, y, max_range))
-    return rl::rm_march_idx8<COUNT>(m, lut, box, max_range, x, y, c, s, cnt);
   return rl::rm_march<COUNT>(m.edt, m.width, m.height, max_range, x, y, c, s, cnt);
 }
 
@@ -278,7 +278,7 @@ __device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, con
 template <int METHOD>
 __device__ __forceinline__ void stage_lut(const MapDev& m, float* sh) {
   if (METHOD == RL_METHOD_RM && m.idx8) {
-    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) sh[i] = m.lut[i];
+    for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) sh[i] = m.steps[i];
     __syncthreads();
   }
 }
@@ -720,7 +720,7 @@ struct rl_ctx {
   hipStream_t stream = nullptr;
   void* trig_buf = nullptr;
   TrigDev trig{};
-  float* d_lut = nullptr;  // RL_LUT_SIZE tabulated distances (see rl_device.h)
+  float* d_lut = nullptr;  // RL_LUT_SIZE tabulated distances, then their march steps (see rl_device.h)
   int xcd_sched = 0;          // RL_OPT_XCD_SCHED
   int lds_box = 0;            // RL_OPT_LDS_BOX
   // the caller's handle plus one per map / batch created on this context: the
@@ -775,7 +775,7 @@ struct rl_map {
   float cddt_td_div_2pi = 0.0f, cddt_ms = 0.0f;
 
   MapDev dev() const {
-    return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut, width, height, nby, ctx->lds_box, world,
+    return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut + RL_LUT_SIZE, width, height, nby, ctx->lds_box, world,
                   d_glt, glt_td, glt_R, glt_scale, glt_td_div_2pi,
                   rl::CddtDev{cddt_width, cddt_trans, cddt_cos, cddt_sin, cddt_bin_off, cddt_start, cddt_vals,
                               cddt_td, cddt_td_div_2pi}};
@@ -965,6 +965,7 @@ int build_compact(rl_map* m, unsigned flags) {
   const int W = m->width, H = m->height;
   m->nby = (H + 7) / 8;
   const size_t nbytes = (size_t)((W + 7) / 8) * (size_t)m->nby * 64;
+  if (nbytes > 0xFFFFFFFFull) return RL_OK;  // 32-bit offsets only: such maps march the fp32 map
   if (hipMalloc(&m->d_idx8, nbytes) != hipSuccess) {
     (void)hipGetLastError();
     m->d_idx8 = nullptr;
@@ -1115,11 +1116,17 @@ int rl_create(int device, rl_ctx** out) {
     std::vector<char> is_sum(1024, 0);
     for (int a = 0; a * a < 1024; ++a)
       for (int b = a; a * a + b * b < 1024; ++b) is_sum[a * a + b * b] = 1;
-    float lut[RL_LUT_SIZE];
+    // lut[0, RL_LUT_SIZE): the distances; lut[RL_LUT_SIZE + i]: the march step
+    // from distance lut[i], as the march computes it
+    float lut[2 * RL_LUT_SIZE];
     int k = 0;
     for (int n = 0; n < 1024 && k < (int)RL_IDX8_ESCAPE; ++n)
       if (is_sum[n]) lut[k++] = sqrtf((float)n);
     for (; k < RL_LUT_SIZE; ++k) lut[k] = -1.0f;  // never matches a distance
+    for (int i = 0; i < RL_LUT_SIZE; ++i) {
+      const float st = lut[i] * 0.999f;
+      lut[RL_LUT_SIZE + i] = (st < 1.0f) ? 1.0f : st;
+    }
     if (hipMalloc(&c->d_lut, sizeof(lut)) != hipSuccess ||
         hipMemcpy(c->d_lut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) {
       (void)hipGetLastError();

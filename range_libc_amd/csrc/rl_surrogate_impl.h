@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
                                                           unsigned* __restrict__ stats, int n) {
   __shared__ float lut[RL_LUT_SIZE];
   __shared__ unsigned cnt[4];
-  for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = m.lut[i];
+  for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = m.steps[i];
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

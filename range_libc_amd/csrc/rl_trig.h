@@ -28,7 +28,9 @@
 //      single load (exceptions cluster: up to ~350 per bucket).
 //
 // Outside |x| <= 64 the result is the double value rounded to nearest, which
-// agrees with glibc except on roughly 0.1% of inputs.
+// agrees with glibc except on roughly 0.1% of inputs while the reduction is
+// exact, i.e. for |x| < 2^20 * pi/2 (about 1.6e6).  Beyond that n * PIO2_1 is
+// no longer exact and the result can differ from glibc's in the last place.
 #pragma once
 #include <stdint.h>
 #include <math.h>

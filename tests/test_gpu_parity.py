@@ -223,6 +223,35 @@ def test_empty_batches(rect_map):
     meth.calc_range_repeat_angles(np.zeros((0, 3), np.float32), np.zeros(4, np.float32), np.zeros(0, np.float32))
 
 
+@pytest.mark.parametrize("flags", [0, "no_compact"])
+def test_rm_special_headings_and_origins(ctx, rect_map, flags):
+    """NaN and infinite headings march nowhere (the reference's float-to-int
+    conversion of a NaN position is INT_MIN, outside the map: max_range); huge,
+    signed-zero and wrapped headings, origins just below 0, on the border, NaN and
+    outside the map, and max_range <= 0, all bit-exact with read counts."""
+    flags = rl._lib.RL_MAP_NO_COMPACT if flags == "no_compact" else 0
+    m = rl.DeviceMap(ctx, O.xmajor(rect_map), flags)
+    # |heading| < 1.6e6: the range where sinf/cosf follow the C library (see rl_trig.h)
+    heads = np.array([np.nan, np.inf, -np.inf, 1e6, -777777.0, 3e4, -3e4, 2 * np.pi, -0.0, 0.0, 1e-30, 4.0],
+                     np.float32)
+    origins = np.array([[100.5, 200.25], [-0.5, 30.0], [30.0, -0.75], [0.0, 0.0], [511.9, 511.9], [512.0, 3.0],
+                        [np.nan, 5.0], [5.0, np.inf], [-3.0, 40.0], [250.0, 250.0]], np.float32)
+    q = np.array([[x, y, h] for x, y in origins for h in heads], np.float32)
+    n = q.shape[0]
+    for R in (500.0, 0.0, -1.0):
+        out = np.empty(n, np.float32)
+        m.calc_range_many(rl.RL_METHOD_RM, R, q, out, n)
+        want, want_cnt = O.Oracle(rect_map, R, "rm").calc_range_many(q, counts=True)
+        assert_bits_equal(out, want, f"R={R}")
+    torch = _torch()
+    d_q = torch.from_numpy(q).cuda()
+    cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    m.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_GRID, 500.0, d_q.data_ptr(), 0, cnt.data_ptr(), n, 1)
+    ctx.sync()
+    _, want_cnt = O.Oracle(rect_map, 500.0, "rm").calc_range_many(q, counts=True)
+    assert np.array_equal(cnt.cpu().numpy(), want_cnt)
+
+
 # --------------------------------------------------------------------------- step counts
 
 def test_step_counts_match_oracle(corridor_map):
