@@ -56,14 +56,15 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
     const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
     const int p = item.first + wib * per_wave + seg;
     const int pp = p < bm.N ? p : -1;
+    const BeamIn none{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, false};
     if (bm.sensor)
       particle_beams<METHOD, true>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
                                    bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                   boxes + wib * RL_BOX_BYTES);
+                                   boxes + wib * RL_BOX_BYTES, none);
     else
       particle_beams<METHOD, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
                                     bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                    boxes + wib * RL_BOX_BYTES);
+                                    boxes + wib * RL_BOX_BYTES, none);
   }
 }
 

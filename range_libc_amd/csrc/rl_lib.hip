@@ -441,6 +441,30 @@ __global__ __launch_bounds__(1024) void k_region(MapDev m, const float* __restri
   }
 }
 
+// A particle's pose and its first beam's angle and observation, loaded before
+// the kernel stages its tables so that the loads overlap (ok = false: not
+// loaded, particle_beams loads them itself).
+struct BeamIn {
+  float x, y, th, a0, o0;
+  bool ok;
+};
+
+template <bool SENSOR>
+__device__ __forceinline__ BeamIn load_beam_in(const float* __restrict__ ins, const float* __restrict__ angles,
+                                               const float* __restrict__ obs, int p, int M, int j0) {
+  BeamIn b{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, true};
+  if (p >= 0) {
+    b.x = ins[3 * (size_t)p];
+    b.y = ins[3 * (size_t)p + 1];
+    b.th = ins[3 * (size_t)p + 2];
+    if (j0 < M) {
+      b.a0 = angles[j0];
+      if (SENSOR) b.o0 = obs[j0];
+    }
+  }
+  return b;
+}
+
 // All beams of particle p (p < 0: none) on the lanes of one wave segment:
 // numpy_calc_range_angles (RangeLib.h:482-520) and, with SENSOR,
 // eval_sensor_model (RangeLib.h:533-555) on the ranges just written -- each
@@ -455,13 +479,16 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
                                                const double* __restrict__ table, int K,
                                                float* __restrict__ ranges,
                                                double* __restrict__ weights, int p, int M, int S,
-                                               int j0, int base, uint8_t* box_lds) {
+                                               int j0, int base, uint8_t* box_lds, BeamIn pre) {
   const bool valid = p >= 0;
   const float hi = (float)((double)(float)K - 1.0);
   float gx = 0.0f, gy = 0.0f, theta = 0.0f;
   if (valid) {
-    rl::to_grid(m.world, ins[3 * (size_t)p], ins[3 * (size_t)p + 1], &gx, &gy);
-    theta = -ins[3 * (size_t)p + 2] + m.world.rotation_const;
+    const float wx = pre.ok ? pre.x : ins[3 * (size_t)p];
+    const float wy = pre.ok ? pre.y : ins[3 * (size_t)p + 1];
+    const float wth = pre.ok ? pre.th : ins[3 * (size_t)p + 2];
+    rl::to_grid(m.world, wx, wy, &gx, &gy);
+    theta = -wth + m.world.rotation_const;
   }
   // one particle per wave: its beams share the LDS box around the origin
   rl::Box box{nullptr, 0, 0};
@@ -473,11 +500,13 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
     double term = 1.0;
     if (valid && j < M) {
       int unused;
-      float d = cast_one<METHOD, false>(m, lut, box, trig, max_range, gy, gx, theta - angles[j], &unused) *
+      const bool first = c0 == 0 && pre.ok;
+      const float aj = first ? pre.a0 : angles[j];
+      float d = cast_one<METHOD, false>(m, lut, box, trig, max_range, gy, gx, theta - aj, &unused) *
                 m.world.scale;
       ranges[(size_t)p * M + j] = d;
       if (SENSOR) {
-        float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
+        float r = rl::clamp_table((first ? pre.o0 : obs[j]) * m.world.inv_scale, hi);
         float dd = rl::clamp_table(d * m.world.inv_scale, hi);
         term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)dd];
       }
@@ -496,16 +525,17 @@ __global__ __launch_bounds__(256) void k_particles(MapDev m, TrigDev trig, float
                                                    const double* __restrict__ table, int K,
                                                    float* __restrict__ ranges,
                                                    double* __restrict__ weights, int N, int M, int S) {
-  __shared__ float lut[RL_LUT_SIZE];
-  stage_lut<METHOD>(m, lut);
   const int lane = threadIdx.x & 63;
   const int wave = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int per_wave = 64 / S;
   const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
-  int p = wave * per_wave + seg;
+  const int p = wave * per_wave + seg < N ? wave * per_wave + seg : -1;
+  const BeamIn pre = load_beam_in<SENSOR>(ins, angles, obs, p, M, j0);
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
-  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights,
-                                 p < N ? p : -1, M, S, j0, base, boxes + (threadIdx.x >> 6) * RL_BOX_BYTES);
+  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights, p, M, S,
+                                 j0, base, boxes + (threadIdx.x >> 6) * RL_BOX_BYTES, pre);
 }
 
 // Particles of one region chunk per workgroup (see Sched).
@@ -524,16 +554,17 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
   const long long first = (long long)(blockIdx.x / RL_REGIONS) * per_block;
   const int cnt = sched.count[r];
   if (first >= cnt) return;  // uniform across the workgroup
-  __shared__ float lut[RL_LUT_SIZE];
-  stage_lut<METHOD>(m, lut);
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
   const int seg = lane / S, j0 = lane - seg * S, base = seg * S;
   const long long idx = first + wib * per_wave + seg;
   const int p = idx < cnt ? sched.perm[sched.offset[r] + idx] : -1;
+  const BeamIn pre = load_beam_in<SENSOR>(ins, angles, obs, p, M, j0);
+  __shared__ float lut[RL_LUT_SIZE];
+  stage_lut<METHOD>(m, lut);
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
-  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights,
-                                 p, M, S, j0, base, boxes + wib * RL_BOX_BYTES);
+  particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights, p, M, S,
+                                 j0, base, boxes + wib * RL_BOX_BYTES, pre);
 }
 
 // GiantLUTCast table (RangeLib.h:1781-1816): entry k = (cell * td + i), cell =
