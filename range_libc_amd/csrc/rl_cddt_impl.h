@@ -125,19 +125,24 @@ __global__ void k_cddt_compact(long long nb, const long long* __restrict__ start
   if (!COPY) kept[b] = k;
 }
 
-// 5. Prune marks (RangeLib.h:1172-1275), one thread per (cell, angle).  Where
-// the reference's linear search leaves its index unset (no entry at or
-// beyond the cell), the lower_bound value is taken, as its binary search.
-__global__ void k_cddt_prune_mark(const uint8_t* __restrict__ occ, int W, int H, int nang, float max_range,
-                                  const int* __restrict__ width, const float* __restrict__ trans,
-                                  const float* __restrict__ cosv, const float* __restrict__ sinv,
-                                  const long long* __restrict__ bin_off, const long long* __restrict__ start,
-                                  const float* __restrict__ vals, uint8_t* __restrict__ used) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)W * H * nang) return;
-  const long long cell = i / nang;
-  const int a = (int)(i - cell * nang);
-  const int x = (int)(cell / H), y = (int)(cell - (long long)x * H);
+// 5. Prune marks (RangeLib.h:1172-1275), one thread per (cell, angle): angle
+// a = blockIdx.y, cells x-major along blockIdx.x, so the lanes of a wave are
+// neighbouring cells of one column under one angle and look up neighbouring
+// bins.  Where the reference's linear search leaves its index unset (no entry
+// at or beyond the cell), the lower_bound value is taken, as its binary search.
+// W * H < 2^31 (checked when the tables are built).
+__global__ __launch_bounds__(256) void k_cddt_prune_mark(const uint8_t* __restrict__ occ, int W, int H, float max_range,
+                                                         const int* __restrict__ width, const float* __restrict__ trans,
+                                                         const float* __restrict__ cosv, const float* __restrict__ sinv,
+                                                         const long long* __restrict__ bin_off,
+                                                         const long long* __restrict__ start,
+                                                         const float* __restrict__ vals, uint8_t* __restrict__ used,
+                                                         int a0) {
+  const unsigned cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= (unsigned)W * (unsigned)H) return;
+  if (occ[cell]) return;  // (every exit below leaves no mark, so the order of the tests is free)
+  const int a = a0 + (int)blockIdx.y;
+  const int x = (int)(cell / (unsigned)H), y = (int)(cell - (unsigned)x * (unsigned)H);
   const float fx = (float)(0.5 + x), fy = (float)(0.5 + y);
   const float cs = cosv[a], sn = sinv[a];
   const float lx = fx * cs - fy * sn;
@@ -145,22 +150,22 @@ __global__ void k_cddt_prune_mark(const uint8_t* __restrict__ occ, int W, int H,
   const unsigned li = (unsigned)rl::trunc_x86(ly);
   if (li >= (unsigned)width[a]) return;
   const long long b = bin_off[a] + li, s = start[b];
-  const long long n = start[b + 1] - s;
+  const int n = (int)(start[b + 1] - s);  // fewer than 2^31 entries in all (checked at build)
   const float* v = vals + s;
-  const long long high = n - 1;
+  uint8_t* u = used + s;
+  const int high = n - 1;
   if (high == -1) return;
-  if (occ[cell]) return;
   if (v[high] < lx && lx - v[high] < max_range) {
-    used[s + high] = 1;
+    u[high] = 1;
     return;
   }
-  long long lo = 0, hi = n;  // lower_bound
+  int lo = 0, hi = n;  // lower_bound
   while (lo < hi) {
-    const long long mid = (lo + hi) >> 1;
+    const int mid = (lo + hi) >> 1;
     if (v[mid] < lx) lo = mid + 1; else hi = mid;
   }
-  if (lo < n) used[s + lo] = 1;
-  if (lo >= 1) used[s + lo - 1] = 1;
+  if (lo < n) u[lo] = 1;
+  if (lo >= 1) u[lo - 1] = 1;
 }
 
 // RangeMethod::calc_range_many_radial_optimized (RangeLib.h:616-676), one
@@ -379,12 +384,13 @@ int cddt_prune_locked(rl_map* m, float max_range) {
   DevBuf used;
   TRY(used.alloc((size_t)m->cddt_nv + 1));
   HIP_TRY(hipMemsetAsync(used.p, 0, (size_t)m->cddt_nv + 1, st));
-  const long long total = (long long)W * H * m->cddt_nang;
-  if (total > 0) {
-    hipLaunchKernelGGL(k_cddt_prune_mark, dim3(grid_for(total, 256)), dim3(256), 0, st, (const uint8_t*)m->d_occ,
-                       W, H, m->cddt_nang, max_range, (const int*)m->cddt_width, (const float*)m->cddt_trans,
+  const long long cells = (long long)W * H;
+  for (int a0 = 0; cells > 0 && a0 < m->cddt_nang; a0 += 65535) {  // grid rows: at most 65535 angles per launch
+    const int na = std::min(65535, m->cddt_nang - a0);
+    hipLaunchKernelGGL(k_cddt_prune_mark, dim3(grid_for(cells, 256), (unsigned)na), dim3(256), 0, st,
+                       (const uint8_t*)m->d_occ, W, H, max_range, (const int*)m->cddt_width, (const float*)m->cddt_trans,
                        (const float*)m->cddt_cos, (const float*)m->cddt_sin, (const long long*)m->cddt_bin_off,
-                       (const long long*)m->cddt_start, (const float*)m->cddt_vals, used.as<uint8_t>());
+                       (const long long*)m->cddt_start, (const float*)m->cddt_vals, used.as<uint8_t>(), a0);
     HIP_TRY(hipGetLastError());
   }
   return cddt_compact(m, 1, used.as<uint8_t>());
