@@ -82,20 +82,22 @@ __device__ __forceinline__ CddtProj cddt_project(int cell, int H, float cs, floa
 }
 
 // 3. Counting (fill == false) or filling (fill == true) pass; one thread per
-// (edge cell, angle).  Bins outside [0, width) are skipped (the reference
-// would write outside its table; it does not happen for maps whose edge
-// cells project inside the rotated bounding box, which the translation and
-// width are built to cover).
+// (edge cell, angle): angle a0 + blockIdx.y, edge cells along blockIdx.x (the
+// edge list holds runs of neighbouring cells, one run per wave of
+// k_cddt_edges), so a wave's lanes mostly project nearby cells under one angle.
+// Bins outside [0, width) are skipped (the reference would write outside its
+// table; it does not happen for maps whose edge cells project inside the
+// rotated bounding box, which the translation and width are built to cover).
 template <bool FILL>
-__global__ void k_cddt_scatter(const int* __restrict__ edges, long long n_pairs, int nang, int H,
-                               const int* __restrict__ width, const float* __restrict__ trans,
-                               const float* __restrict__ cosv, const float* __restrict__ sinv,
-                               const long long* __restrict__ bin_off, unsigned long long* __restrict__ counter,
-                               float* __restrict__ vals) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_pairs) return;
-  const long long e = i / nang;
-  const int a = (int)(i - e * nang);
+__global__ __launch_bounds__(256) void k_cddt_scatter(const int* __restrict__ edges, int n_edges, int a0, int H,
+                                                      const int* __restrict__ width, const float* __restrict__ trans,
+                                                      const float* __restrict__ cosv, const float* __restrict__ sinv,
+                                                      const long long* __restrict__ bin_off,
+                                                      unsigned long long* __restrict__ counter,
+                                                      float* __restrict__ vals) {
+  const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= n_edges) return;
+  const int a = a0 + (int)blockIdx.y;
   const CddtProj p = cddt_project(edges[e], H, cosv[a], sinv[a], trans[a]);
   for (int b = p.lower; b <= p.upper; ++b) {
     if (b < 0 || b >= width[a]) continue;
@@ -340,11 +342,13 @@ int cddt_build_locked(rl_map* m, int td) {
   HIP_TRY(hipMemsetAsync(counter.p, 0, (size_t)(nb + 1) * sizeof(unsigned long long), st));
   const long long pairs = (long long)ne * nang;
   if (pairs > 0) {
-    hipLaunchKernelGGL(k_cddt_scatter<false>, dim3(grid_for(pairs, 256)), dim3(256), 0, st, (const int*)edges.p,
-                       pairs, nang, H, (const int*)m->cddt_width, (const float*)m->cddt_trans,
-                       (const float*)m->cddt_cos, (const float*)m->cddt_sin, (const long long*)m->cddt_bin_off,
-                       counter.as<unsigned long long>(), (float*)nullptr);
-    HIP_TRY(hipGetLastError());
+    for (int a0 = 0; a0 < nang; a0 += 65535) {  // grid rows: at most 65535 angles per launch
+      hipLaunchKernelGGL(k_cddt_scatter<false>, dim3(grid_for(ne, 256), (unsigned)std::min(65535, nang - a0)),
+                         dim3(256), 0, st, (const int*)edges.p, (int)ne, a0, H, (const int*)m->cddt_width,
+                         (const float*)m->cddt_trans, (const float*)m->cddt_cos, (const float*)m->cddt_sin,
+                         (const long long*)m->cddt_bin_off, counter.as<unsigned long long>(), (float*)nullptr);
+      HIP_TRY(hipGetLastError());
+    }
   }
   HIP_TRY(hipMalloc(&m->cddt_start, (size_t)(nb + 1) * sizeof(long long)));
   TRY(scan_counts(st, counter.as<unsigned long long>(), m->cddt_start, nb));
@@ -355,11 +359,13 @@ int cddt_build_locked(rl_map* m, int td) {
   DevBuf raw;
   TRY(raw.alloc((size_t)total * sizeof(float)));
   if (pairs > 0) {
-    hipLaunchKernelGGL(k_cddt_scatter<true>, dim3(grid_for(pairs, 256)), dim3(256), 0, st, (const int*)edges.p,
-                       pairs, nang, H, (const int*)m->cddt_width, (const float*)m->cddt_trans,
-                       (const float*)m->cddt_cos, (const float*)m->cddt_sin, (const long long*)m->cddt_bin_off,
-                       counter.as<unsigned long long>(), raw.as<float>());
-    HIP_TRY(hipGetLastError());
+    for (int a0 = 0; a0 < nang; a0 += 65535) {  // grid rows: at most 65535 angles per launch
+      hipLaunchKernelGGL(k_cddt_scatter<true>, dim3(grid_for(ne, 256), (unsigned)std::min(65535, nang - a0)),
+                         dim3(256), 0, st, (const int*)edges.p, (int)ne, a0, H, (const int*)m->cddt_width,
+                         (const float*)m->cddt_trans, (const float*)m->cddt_cos, (const float*)m->cddt_sin,
+                         (const long long*)m->cddt_bin_off, counter.as<unsigned long long>(), raw.as<float>());
+      HIP_TRY(hipGetLastError());
+    }
   }
   // sort each bin
   HIP_TRY(hipMalloc(&m->cddt_vals, (size_t)(total ? total : 1) * sizeof(float)));
