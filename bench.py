@@ -307,6 +307,89 @@ def bench_glt(args):
     print(json.dumps(result))
 
 
+def bench_bresenham(args):
+    """Secondary workload: Bresenham calc_range against ray marching on the
+    particle-filter step's map (2048x2048 corridors): 1 M queries from integer
+    free cells, headings uniform in [0, 2 pi), max_range 1000, through
+    numpy_calc_range (world frame, identity parameters).  Both methods are
+    timed on the same device-resident queries; Bresenham's algorithmic bytes
+    are one occupancy byte per visited cell (counted on the device) plus 16 B of
+    pose and range per ray.  A sample of rays is checked against the oracle and
+    every mismatch counted; the reference's CPU path is timed on a sample of
+    the same queries over the host's cores."""
+    import torch
+
+    import range_libc_amd as rl
+    from range_libc_amd import workloads as W
+
+    NQ, R = 1_000_000, 1000.0
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    img = W.corridor_map(MAP_SIDE, 1)
+    q = W.world_poses(img, NQ, seed=7, integer=True)
+    m = rl.DeviceMap(ctx, np.ascontiguousarray(img.T.astype(np.uint8)))
+    d_q = torch.from_numpy(q).to(dev)
+    outs = {k: torch.empty(NQ, dtype=torch.float32, device=dev) for k in ("bl", "rm")}
+    meth = {"bl": rl.RL_METHOD_BL, "rm": rl.RL_METHOD_RM}
+    cells = {}
+    for k in meth:
+        cnt = torch.zeros(NQ, dtype=torch.int32, device=dev)
+        m.count_steps_dev(meth[k], rl.RL_FRAME_WORLD, R, d_q.data_ptr(), 0, cnt.data_ptr(), NQ)
+        stream.synchronize()
+        cells[k] = int(cnt.sum().item())
+
+    def timed(k, reps):
+        fn = lambda: m.numpy_calc_range_dev(meth[k], R, d_q.data_ptr(), outs[k].data_ptr(), NQ)  # noqa: E731
+        fn()
+        torch.cuda._sleep(50_000_000)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    reps = max(5, args.steps // 10)
+    ms = {k: timed(k, reps) for k in meth}
+    alg = {"bl": cells["bl"] + 16 * NQ, "rm": 4 * cells["rm"] + 16 * NQ}
+    result = {
+        "metric": "Bresenham rays/s on 1 GPU (1 M integer-cell queries, 2048^2 corridors, R=1000)",
+        "value": NQ / (ms["bl"] * 1e-3), "unit": "rays/s", "higher_is_better": True,
+        "bl_ms": ms["bl"], "rm_ms": ms["rm"], "rm_rays_per_s": NQ / (ms["rm"] * 1e-3),
+        "bl_cells_per_ray": cells["bl"] / NQ, "rm_reads_per_ray": cells["rm"] / NQ,
+        "roofline": {"bound": "hbm", "achieved": alg["bl"] / (ms["bl"] * 1e-3) / 1e9, "peak": 8000.0,
+                     "unit": "GB/s", "frac": alg["bl"] / (ms["bl"] * 1e-3) / 1e9 / 8000.0,
+                     "kernel": "k_cast<BL, world>", "alg_bytes_per_launch": alg["bl"],
+                     "visited_cells_per_s": cells["bl"] / (ms["bl"] * 1e-3)},
+        "config": {"workload": "bresenham_vs_rm_corridor2048_1m_r1000"}}
+    from oracle import oracle as O
+
+    pick = np.random.default_rng(0).choice(NQ, 4000, replace=False)
+    for k in meth:
+        want = O.Oracle(img, R, k).numpy_calc_range(q[pick])
+        got = outs[k].cpu().numpy()[pick]
+        result[f"{k}_sample_mismatches"] = int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+    result["sample_rays_checked"] = int(pick.size)
+    if not args.no_cpu and O.ref_available():
+        threads = len(os.sched_getaffinity(0))
+        ref = O.Reference(img, R, "bl")
+        ref.numpy_calc_range(q[:1000], threads)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_budget, 5.0) or reps == 0:
+            ref.numpy_calc_range(q, threads)
+            reps += 1
+        dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": reps * NQ / dt, "unit": "rays/s", "cores": threads,
+                                  "kind": "reference",
+                                  "sample": f"BresenhamsLine numpy_calc_range over all {NQ} queries, {reps} passes "
+                                            f"in {dt:.3f} s"}
+    print(json.dumps(result))
+
+
 def bench_cddt(args):
     """Secondary workload: CDDTCast / PCDDT on the particle-filter step's map
     (2048x2048 corridors, max_range 500, 108 heading bins): tables built and
@@ -392,7 +475,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
-    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt", "cddt"],
+    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt", "cddt", "bresenham"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
                          "surrogate (learned start point on the step's map)")
     ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
@@ -416,6 +499,8 @@ def main():
         return bench_glt(args)
     if args.workload == "cddt":
         return bench_cddt(args)
+    if args.workload == "bresenham":
+        return bench_bresenham(args)
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)

@@ -252,6 +252,28 @@ def test_rm_special_headings_and_origins(ctx, rect_map, flags):
     assert np.array_equal(cnt.cpu().numpy(), want_cnt)
 
 
+def test_bresenham_vs_rm_full_size(corridor_map):
+    """The Bresenham-vs-ray-marching workload at full size: 1 M queries from
+    integer free cells of the 2048^2 corridor map, max_range 1000.  Every ray is
+    cast on the GPU; 20,000 of them, drawn at random, are compared bit for bit
+    with the oracle, and all of them satisfy 0 <= r <= R + 2 (a hit cell's
+    distance can exceed R by less than the cell diagonal)."""
+    torch = _torch()
+    ctx = rl.default_context(0)
+    n, R = 1_000_000, 1000.0
+    q = W.world_poses(corridor_map, n, seed=7, integer=True)
+    m = rl.DeviceMap(ctx, O.xmajor(corridor_map))
+    d_q = torch.from_numpy(q).cuda()
+    pick = np.random.default_rng(3).choice(n, 20_000, replace=False)
+    for method, kind in ((rl.RL_METHOD_BL, "bl"), (rl.RL_METHOD_RM, "rm")):
+        out = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+        m.numpy_calc_range_dev(method, R, d_q.data_ptr(), out.data_ptr(), n)
+        ctx.sync()
+        got = out.cpu().numpy()
+        assert np.all(np.isfinite(got)) and got.min() >= 0.0 and got.max() <= R + 2.0, kind
+        assert_bits_equal(got[pick], O.Oracle(corridor_map, R, kind).numpy_calc_range(q[pick]), f"{kind} sample")
+
+
 # --------------------------------------------------------------------------- step counts
 
 def test_step_counts_match_oracle(corridor_map):
