@@ -162,10 +162,11 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
 }
 
 // Speculation width and start.  Measured on a 2048^2 corridor map, 4000
-// particles x 60 beams: plain 39.8 us per step; width 2 after 8 reads 37.1 us;
-// widths 2-4 after 4-24 reads 37.1-38.0 us.  Earlier speculation doubles the
-// loads of the bulk of the launch; later speculation leaves the long rays'
-// chains (runs of equal steps along corridor walls) unshortened.
+// particles x 60 beams, with the single-exit loops: plain 29.7 us per step;
+// width 2 after 8 or 12 reads 25.1, 25.2 us; width 3 after 4, 8 or 12 reads
+// 25.7, 25.0, 24.9 us.  Earlier speculation
+// doubles the loads of the bulk of the launch; later speculation leaves the
+// long rays' chains (runs of equal steps along corridor walls) unshortened.
 #ifndef RL_SPEC
 #define RL_SPEC 2  // positions loaded per round of the speculative march (1: plain)
 #endif
@@ -205,15 +206,21 @@ __device__ __forceinline__ float idx8_step(const MapDev& m, const float* lut, un
 // plain truncation.  Identical results to rm_march: a cell's code is 0 exactly
 // when its distance is 0, and any other code maps back to the exact distance.
 //
-// Optional speculation (RL_SPEC > 1), after a ray's first RL_SPEC_AFTER reads:
-// each round loads the cells at t, t + g, (t + g) + g, ... (RL_SPEC
-// positions, all loads issued before any is used), where g is the last step
-// taken, and then walks them in order: position k + 1 is the true next
-// position exactly when cell k's step equals g, i.e. when its code equals the
-// code that gave g (distinct codes have distinct steps; escaped cells never
-// match).  The walk stops at the first mismatch and the next round starts from
-// the true next t.  Positions, float operations and results are those of the
-// plain loop, and `reads` counts resolved positions only.
+// Each loop has one exit per round: the ways a ray ends (past max_range, off
+// the map, on an occupied cell) are decided with selects, and only the kind of
+// end (1 = max_range, 2 = hit) and its cell leave the loop; the exec-mask
+// bookkeeping of three divergent exits per step cost a quarter of the kernel.
+// Off-map positions load cell (0, 0), which every map has, and ignore it.
+//
+// Speculation (RL_SPEC > 1), after a ray's first RL_SPEC_AFTER reads: each
+// round loads the cells at t, t + g, (t + g) + g, ... (RL_SPEC positions, all
+// loads issued before any is used), where g is the last step taken, and then
+// walks them in order: position k + 1 is the true next position exactly when
+// cell k's step equals g, i.e. when its code equals the code that gave g
+// (distinct codes have distinct steps; escaped cells never match).  The walk
+// stops at the first mismatch and the next round starts from the true next t.
+// Positions, float operations and results are those of the plain loop, and
+// `reads` counts resolved positions only.
 template <bool COUNT, bool BOX>
 __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* lut, const Box& box,
                                                  float max_range, float x0, float y0, float dx,
@@ -222,71 +229,117 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   // (t, g, gcode, nr): position, last step, the code that gave it (256 matches
   // no cell) and reads so far; (0, 1, 1, 0) for a march from its origin
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
-  for (int i = nr; (RL_SPEC == 1 || i < RL_SPEC_AFTER) && t < max_range; ++i) {
-    int px = (int)(x0 + dx * t);
-    int py = (int)(y0 + dy * t);
-    if ((unsigned)px >= W || (unsigned)py >= H) {
-      if (COUNT) *reads = nr;
-      return max_range;
-    }
-    const unsigned c = idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py);
-    if (COUNT) ++nr;
-    if (c == 0u) {
-      float xd = (float)px - x0;
-      float yd = (float)py - y0;
-      if (COUNT) *reads = nr;
-      return __builtin_sqrtf(xd * xd + yd * yd);
+  int end = 0, hx = 0, hy = 0;
+  for (int i = nr; RL_SPEC == 1 || i < RL_SPEC_AFTER; ++i) {
+    const int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
+    const bool live = t < max_range;
+    const bool inb = (unsigned)px < W && (unsigned)py < H;
+    const unsigned c = idx8_code<BOX>(m, box, inb ? (unsigned)px : 0u, inb ? (unsigned)py : 0u);
+    const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
+    if (COUNT) nr += (live && inb) ? 1 : 0;
+    if (e) {
+      end = e, hx = px, hy = py;
+      break;
     }
     g = idx8_step(m, lut, c, px, py);
     gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
     t += g;
   }
-  if (RL_SPEC > 1) {
-    while (t < max_range) {
-      float tk[RL_SPEC];
-      int px[RL_SPEC], py[RL_SPEC];
-      unsigned code[RL_SPEC];
-      bool inb[RL_SPEC];
-#pragma unroll
-      for (int k = 0; k < RL_SPEC; ++k) {
-        tk[k] = (k == 0) ? t : tk[k - 1] + g;
-        px[k] = (int)(x0 + dx * tk[k]);
-        py[k] = (int)(y0 + dy * tk[k]);
-        inb[k] = (unsigned)px[k] < W && (unsigned)py[k] < H;
-      }
-#pragma unroll
-      for (int k = 0; k < RL_SPEC; ++k)
-        code[k] = idx8_code<BOX>(m, box, inb[k] ? (unsigned)px[k] : 0u, inb[k] ? (unsigned)py[k] : 0u);
-#pragma unroll
-      for (int k = 0; k < RL_SPEC; ++k) {
-        // position k is the true position tk[k], and tk[k] < max_range
-        if (!inb[k]) {
-          if (COUNT) *reads = nr;
-          return max_range;
-        }
-        if (COUNT) ++nr;
-        const unsigned c = code[k];
-        if (c == 0u) {
-          float xd = (float)px[k] - x0;
-          float yd = (float)py[k] - y0;
-          if (COUNT) *reads = nr;
-          return __builtin_sqrtf(xd * xd + yd * yd);
-        }
-        if (c != gcode || k == RL_SPEC - 1) {
-          g = idx8_step(m, lut, c, px[k], py[k]);
-          gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
-          t = tk[k] + g;
-          break;
-        }
-        // same step as the last: the next true position is tk[k + 1]
-        if (!(tk[k + 1 < RL_SPEC ? k + 1 : k] < max_range)) {
-          if (COUNT) *reads = nr;
-          return max_range;
-        }
-      }
+  while (RL_SPEC == 2 && !end) {  // width 2, written out
+    const float t0 = t, t1 = t + g;
+    const int px0 = (int)(x0 + dx * t0), py0 = (int)(y0 + dy * t0);
+    const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
+    const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
+    const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
+    const unsigned c0 = idx8_code<BOX>(m, box, in0 ? (unsigned)px0 : 0u, in0 ? (unsigned)py0 : 0u);
+    const unsigned c1 = idx8_code<BOX>(m, box, in1 ? (unsigned)px1 : 0u, in1 ? (unsigned)py1 : 0u);
+    const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
+    const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
+    const bool use1 = e0 == 0 && c0 == gcode;  // position 1 is the true next position
+    const int es = use1 ? e1 : e0;
+    // positions before the resolved one were read and continued; the resolved
+    // one was read unless it is past max_range or off the map (es == 1)
+    if (COUNT) nr += (use1 ? 1 : 0) + (es != 1 ? 1 : 0);
+    const int sx = use1 ? px1 : px0, sy = use1 ? py1 : py0;
+    const unsigned cs = use1 ? c1 : c0;
+    if (es) {
+      end = es, hx = sx, hy = sy;
+      break;
     }
+    g = idx8_step(m, lut, cs, sx, sy);
+    gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
+    t = (use1 ? t1 : t0) + g;
+  }
+  while (RL_SPEC == 3 && !end) {  // width 3, written out
+    const float t0 = t, t1 = t + g, t2 = t1 + g;
+    const int px0 = (int)(x0 + dx * t0), py0 = (int)(y0 + dy * t0);
+    const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
+    const int px2 = (int)(x0 + dx * t2), py2 = (int)(y0 + dy * t2);
+    const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
+    const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
+    const bool in2 = (unsigned)px2 < W && (unsigned)py2 < H;
+    const unsigned c0 = idx8_code<BOX>(m, box, in0 ? (unsigned)px0 : 0u, in0 ? (unsigned)py0 : 0u);
+    const unsigned c1 = idx8_code<BOX>(m, box, in1 ? (unsigned)px1 : 0u, in1 ? (unsigned)py1 : 0u);
+    const unsigned c2 = idx8_code<BOX>(m, box, in2 ? (unsigned)px2 : 0u, in2 ? (unsigned)py2 : 0u);
+    const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
+    const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
+    const int e2 = (!(t2 < max_range) || !in2) ? 1 : (c2 == 0u ? 2 : 0);
+    const bool u1 = e0 == 0 && c0 == gcode;
+    const bool u2 = u1 && e1 == 0 && c1 == gcode;
+    const int es = u2 ? e2 : (u1 ? e1 : e0);
+    if (COUNT) nr += (u1 ? 1 : 0) + (u2 ? 1 : 0) + (es != 1 ? 1 : 0);
+    const int sx = u2 ? px2 : (u1 ? px1 : px0), sy = u2 ? py2 : (u1 ? py1 : py0);
+    const unsigned cs = u2 ? c2 : (u1 ? c1 : c0);
+    if (es) {
+      end = es, hx = sx, hy = sy;
+      break;
+    }
+    g = idx8_step(m, lut, cs, sx, sy);
+    gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
+    t = (u2 ? t2 : (u1 ? t1 : t0)) + g;
+  }
+  while (RL_SPEC > 3 && !end) {  // any width (slower selects than the written-out forms)
+    float tk[RL_SPEC];
+    int px[RL_SPEC], py[RL_SPEC], e[RL_SPEC];
+    unsigned c[RL_SPEC];
+    bool in[RL_SPEC];
+#pragma unroll
+    for (int k = 0; k < RL_SPEC; ++k) {
+      tk[k] = (k == 0) ? t : tk[k - 1] + g;
+      px[k] = (int)(x0 + dx * tk[k]);
+      py[k] = (int)(y0 + dy * tk[k]);
+      in[k] = (unsigned)px[k] < W && (unsigned)py[k] < H;
+    }
+#pragma unroll
+    for (int k = 0; k < RL_SPEC; ++k)
+      c[k] = idx8_code<BOX>(m, box, in[k] ? (unsigned)px[k] : 0u, in[k] ? (unsigned)py[k] : 0u);
+#pragma unroll
+    for (int k = 0; k < RL_SPEC; ++k) e[k] = (!(tk[k] < max_range) || !in[k]) ? 1 : (c[k] == 0u ? 2 : 0);
+    int sel = RL_SPEC - 1;
+#pragma unroll
+    for (int k = RL_SPEC - 2; k >= 0; --k)
+      if (e[k] != 0 || c[k] != gcode) sel = k;
+    int es = e[0], sx = px[0], sy = py[0];
+    unsigned cs = c[0];
+    float ts = tk[0];
+#pragma unroll
+    for (int k = 1; k < RL_SPEC; ++k)
+      if (sel == k) es = e[k], sx = px[k], sy = py[k], cs = c[k], ts = tk[k];
+    if (COUNT) nr += sel + (es != 1 ? 1 : 0);
+    if (es) {
+      end = es, hx = sx, hy = sy;
+      break;
+    }
+    g = idx8_step(m, lut, cs, sx, sy);
+    gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
+    t = ts + g;
   }
   if (COUNT) *reads = nr;
+  if (end == 2) {
+    const float xd = (float)hx - x0;
+    const float yd = (float)hy - y0;
+    return __builtin_sqrtf(xd * xd + yd * yd);
+  }
   return max_range;
 }
 
