@@ -235,15 +235,45 @@ RL_HD long long bitmap_index(int e_min, float x) {
   return (long long)((slot << 23) | (ab & 0x7fffffu));
 }
 
+// neighbours() without its division: returns the nearest float f and the
+// other candidate, and whether |v - midpoint| / gap <= delta.  gap, the
+// spacing of two adjacent floats, is a power of two, so |v - m| <= delta * gap
+// is exactly that test.
+RL_HD float neighbours_near(double v, double delta, float* other_out, bool* near_out) {
+  float f = (float)v;
+  double fd = (double)f;
+  if (fd == v) {
+    *other_out = f;
+    *near_out = 0.5 <= delta;
+    return f;
+  }
+  bool up = v > fd;
+  uint32_t fb = f2u(f);
+  uint32_t ob;
+  if ((fb & 0x7fffffffu) == 0u) {
+    ob = up ? 0x00000001u : 0x80000001u;
+  } else if ((fb >> 31) == 0u) {
+    ob = up ? fb + 1u : fb - 1u;
+  } else {
+    ob = up ? fb - 1u : fb + 1u;
+  }
+  float o = u2f(ob);
+  double od = (double)o;
+  double m = 0.5 * (fd + od);
+  *other_out = o;
+  *near_out = fabs(v - m) <= delta * fabs(od - fd);
+  return f;
+}
+
 RL_HD void sincosf_libc_bm(float h, const Bitmap& bs, const Bitmap& bc, float* s, float* c) {
   double sd, cd;
   sincos_d(h, &sd, &cd);
   float os, oc;
-  double ds, dc;
-  float fs = neighbours(sd, &os, &ds);
-  float fc = neighbours(cd, &oc, &dc);
-  long long is = ds <= bs.delta ? bitmap_index(bs.e_min, h) : -1;
-  long long ic = dc <= bc.delta ? bitmap_index(bc.e_min, h) : -1;
+  bool ns, nc;
+  float fs = neighbours_near(sd, bs.delta, &os, &ns);
+  float fc = neighbours_near(cd, bc.delta, &oc, &nc);
+  long long is = ns ? bitmap_index(bs.e_min, h) : -1;
+  long long ic = nc ? bitmap_index(bc.e_min, h) : -1;
   // both probes are issued before either is consumed
   uint32_t vs = is >= 0 ? (uint32_t)bs.bits[is >> 3] : 0u;
   uint32_t vc = ic >= 0 ? (uint32_t)bc.bits[ic >> 3] : 0u;
