@@ -686,6 +686,11 @@ static int debug_gemm(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C,
 
 }  // namespace
 
+__global__ void k_sg_zero_u32(unsigned* __restrict__ p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 extern "C" {
 
 int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
@@ -897,7 +902,10 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
   TRY(sg_reserve(s, n));
   hipStream_t st = c->stream;
   MapDev md = s->map->dev();
-  if (stats) HIP_TRY(hipMemsetAsync(s->d_stats, 0, 16, st));
+  if (stats) {  // a fill kernel, not a memset node, so that the launch sequence stays graph-safe
+    hipLaunchKernelGGL(k_sg_zero_u32, dim3(1), dim3(64), 0, st, s->d_stats, 4);
+    HIP_TRY(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_sg_rays_from_particles, dim3(grid_for(n, 256)), dim3(256), 0, st, md, c->trig, d_ins,
                      d_angles, N, M, s->d_rays);
   HIP_TRY(hipGetLastError());
