@@ -567,25 +567,49 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
                                  j0, base, boxes + wib * RL_BOX_BYTES, pre);
 }
 
+// GiantLUTCast's ray directions: (cosf, sinf) of angle i * step for i < td, the
+// values every ray of bin i uses (the reference evaluates them per ray; the same
+// function of the same float gives the same bits).
+__global__ void k_glt_dirs(TrigDev trig, int td, float step, float2* __restrict__ dirs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= td) return;
+  float s, c;
+  rl::sincos_libc((float)i * step, trig, &s, &c);
+  dirs[i] = make_float2(c, s);
+}
+
 // GiantLUTCast table (RangeLib.h:1781-1816): entry k = (cell * td + i), cell =
 // x * H + y, is the ray marched from the integer point (x, y) at angle
 // i * step, clamped to max_range and quantized to uint16; `step` and
 // `limits_div_max` are the reference's float constants.  One ray per thread
-// (grid-stride); neighbouring threads share a start cell.
-__global__ __launch_bounds__(256) void k_glt_build(MapDev m, TrigDev trig, float max_range, int td, float step,
-                                                   float limits_div_max, uint16_t* __restrict__ out,
-                                                   long long total) {
+// (grid-stride); neighbouring threads share a start cell.  WIDE: 64-bit entry
+// indices (tables of 2^32 entries or more), else 32-bit index arithmetic.
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_glt_build(MapDev m, float max_range, int td,
+                                                   const float2* __restrict__ dirs, float limits_div_max,
+                                                   uint16_t* __restrict__ out, long long total) {
   __shared__ float lut[RL_LUT_SIZE];
   stage_lut<RL_METHOD_RM>(m, lut);
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
-    const long long cell = k / td;
-    const int i = (int)(k - cell * td);
-    const int x = (int)(cell / m.height), y = (int)(cell - (long long)x * m.height);
-    const float angle = (float)i * step;
+    int x, y, i;
+    if (WIDE) {
+      const long long cell = k / td;
+      i = (int)(k - cell * td);
+      x = (int)(cell / m.height);
+      y = (int)(cell - (long long)x * m.height);
+    } else {
+      const unsigned k32 = (unsigned)k, cell = k32 / (unsigned)td;
+      i = (int)(k32 - cell * (unsigned)td);
+      x = (int)(cell / (unsigned)m.height);
+      y = (int)(cell - (unsigned)x * (unsigned)m.height);
+    }
+    const float2 d = dirs[i];
+    const float fx = (float)x, fy = (float)y;
     int unused;
-    float r = cast_one<RL_METHOD_RM, false>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, (float)x, (float)y,
-                                            angle, &unused);
+    float r = (m.idx8 && rl::fast_march_ok(fx, fy, max_range))
+                  ? rl::rm_march_idx8<false>(m, lut, rl::Box{nullptr, 0, 0}, max_range, fx, fy, d.x, d.y, &unused)
+                  : rl::rm_march<false>(m.edt, m.width, m.height, max_range, fx, fy, d.x, d.y, &unused);
     r = (r < max_range) ? r : max_range;  // std::min(max_range, r)
     out[k] = (uint16_t)(unsigned)(r * limits_div_max);
   }
@@ -1268,15 +1292,24 @@ int rl_map_build_giant_lut(rl_map* m, float max_range, int theta_disc) {
   HIP_TRY(hipEventCreate(&e1));
   long long blocks = (total + 255) / 256;
   if (blocks > (1LL << 20)) blocks = 1LL << 20;
-  hipError_t e = hipEventRecord(e0, c->stream);
+  float2* dirs = nullptr;
+  hipError_t e = hipMalloc(&dirs, (size_t)theta_disc * sizeof(float2));
+  if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_glt_build, dim3((unsigned)blocks), dim3(256), 0, c->stream, m->dev(), c->trig, max_range,
-                       theta_disc, step, limits_div_max, m->d_glt, total);
+    hipLaunchKernelGGL(k_glt_dirs, dim3((unsigned)((theta_disc + 255) / 256)), dim3(256), 0, c->stream, c->trig,
+                       theta_disc, step, dirs);
+    if (total <= 0xFFFFFFFFLL)
+      hipLaunchKernelGGL(k_glt_build<false>, dim3((unsigned)blocks), dim3(256), 0, c->stream, m->dev(), max_range,
+                         theta_disc, (const float2*)dirs, limits_div_max, m->d_glt, total);
+    else
+      hipLaunchKernelGGL(k_glt_build<true>, dim3((unsigned)blocks), dim3(256), 0, c->stream, m->dev(), max_range,
+                         theta_disc, (const float2*)dirs, limits_div_max, m->d_glt, total);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) (void)hipEventElapsedTime(&m->glt_ms, e0, e1);
+  if (dirs) (void)hipFree(dirs);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (e != hipSuccess) {
