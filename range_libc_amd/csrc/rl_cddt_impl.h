@@ -40,10 +40,10 @@ __global__ void k_cddt_consts(TrigDev trig, int td, int W, int H, float m2pi_div
 // 2. Edge cells, compacted into `list` (one atomic per wave).
 __global__ void k_cddt_edges(const uint8_t* __restrict__ occ, int W, int H, int* __restrict__ list,
                              unsigned* __restrict__ n_out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;  // W * H < 2^31 (checked by the caller)
   bool e = false;
-  if (i < (long long)W * H) {
-    const int x = (int)(i / H), y = (int)(i - (long long)x * H);
+  if (i < (unsigned)W * (unsigned)H) {
+    const int x = (int)(i / (unsigned)H), y = (int)(i - (unsigned)x * (unsigned)H);
     if (occ[i]) {
       const int dx[8] = {1, -1, 0, 0, 1, -1, 1, -1}, dy[8] = {0, 0, 1, -1, 1, 1, -1, -1};
       for (int k = 0; k < 8; ++k) {
@@ -184,9 +184,9 @@ __global__ __launch_bounds__(256) void k_radial(MapDev m, TrigDev trig, float ma
                                                 int mpi, int off) {
   __shared__ float lut[RL_LUT_SIZE];
   stage_lut<METHOD>(m, lut);
-  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= (long long)N * nr) return;
-  const int p = (int)(r / nr), s = (int)(r - (long long)p * nr);
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;  // N * nr < 2^31 (checked by the caller)
+  if (r >= (unsigned)N * (unsigned)nr) return;
+  const int p = (int)(r / (unsigned)nr), s = (int)(r - (unsigned)p * (unsigned)nr);
   int kind = 0, a = 0;  // 1: pair r, 2: pair r_inv, 3: single ray
   if (s > mpi && s < off) {
     kind = 3;
