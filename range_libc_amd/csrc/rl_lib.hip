@@ -311,7 +311,21 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
 __device__ __forceinline__ double segment_product(double w, double term, int base, int cnt, int S) {
   if (S == 64) {  // one particle per wave: broadcast with scalar lane reads
     int lo = __double2loint(term), hi = __double2hiint(term);
-    for (int jj = 0; jj < cnt; ++jj)
+    int jj = 0;
+    for (; jj + 4 <= cnt; jj += 4) {  // the lane reads of four terms ahead of their (ordered) products
+      const double t0 = __hiloint2double(__builtin_amdgcn_readlane(hi, jj), __builtin_amdgcn_readlane(lo, jj));
+      const double t1 =
+          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 1), __builtin_amdgcn_readlane(lo, jj + 1));
+      const double t2 =
+          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 2), __builtin_amdgcn_readlane(lo, jj + 2));
+      const double t3 =
+          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 3), __builtin_amdgcn_readlane(lo, jj + 3));
+      w *= t0;
+      w *= t1;
+      w *= t2;
+      w *= t3;
+    }
+    for (; jj < cnt; ++jj)
       w *= __hiloint2double(__builtin_amdgcn_readlane(hi, jj), __builtin_amdgcn_readlane(lo, jj));
     return w;
   }
