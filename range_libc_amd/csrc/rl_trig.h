@@ -13,8 +13,8 @@
 //   1. Evaluate sin/cos in double precision: Cody-Waite reduction by pi/2 with
 //      a three-part constant (the first two parts carry 33 significant bits, so
 //      n*P1 and n*P2 are exact for |n| < 2^20), then Taylor polynomials of
-//      degree 23 (sin) and 24 (cos) on |r| <= pi/4 + eps, evaluated in Horner
-//      form.  Truncation error < 2e-25, so the double result is within a few
+//      degree 17 (sin) and 18 (cos) on |r| <= pi/4 + eps, evaluated in Horner
+//      form.  Truncation error < 1e-19, so the double result is within a few
 //      ulp(double) of the true value.  Only IEEE +,-,*,/ and rint are used and
 //      the file must be built with contraction off, so host and device produce
 //      identical doubles.
@@ -99,11 +99,9 @@ RL_HD void sincos_d(float xf, double* s_out, double* c_out) {
     r = r - n * RLT_PIO2_3;
   }
   double r2 = r * r;
-  // sin r = r + r^3 * P(r^2), P = -1/3! + r^2/5! - ...  (Taylor, to r^23)
-  double ps = -3.868170170630684e-23;
-  ps = ps * r2 + 1.9572941063391263e-20;
-  ps = ps * r2 - 8.22063524662433e-18;
-  ps = ps * r2 + 2.8114572543455206e-15;
+  // sin r = r + r^3 * P(r^2), P = -1/3! + r^2/5! - ...  (Taylor, to r^17: on
+  // |r| <= pi/4 the first dropped term, r^19 / 19!, is below 1e-19)
+  double ps = 2.8114572543455206e-15;
   ps = ps * r2 - 7.647163731819816e-13;
   ps = ps * r2 + 1.6059043836821613e-10;
   ps = ps * r2 - 2.505210838544172e-08;
@@ -112,11 +110,9 @@ RL_HD void sincos_d(float xf, double* s_out, double* c_out) {
   ps = ps * r2 + 0.008333333333333333;
   ps = ps * r2 - 0.16666666666666666;
   double sr = r + (r * r2) * ps;
-  // cos r = 1 - r^2/2 + r^4 * Q(r^2), Q = 1/4! - r^2/6! + ...  (Taylor, to r^24)
-  double pc = 1.6117375710961184e-24;
-  pc = pc * r2 - 8.896791392450574e-22;
-  pc = pc * r2 + 4.110317623312165e-19;
-  pc = pc * r2 - 1.5619206968586225e-16;
+  // cos r = 1 - r^2/2 + r^4 * Q(r^2), Q = 1/4! - r^2/6! + ...  (Taylor, to r^18:
+  // the first dropped term, r^20 / 20!, is below 1e-20)
+  double pc = -1.5619206968586225e-16;
   pc = pc * r2 + 4.779477332387385e-14;
   pc = pc * r2 - 1.1470745597729725e-11;
   pc = pc * r2 + 2.08767569878681e-09;
