@@ -620,7 +620,7 @@ __device__ __forceinline__ bool fast_march_ok(float x0, float y0, float max_rang
 __device__ __forceinline__ bool occupied(const uint8_t* __restrict__ occ, int W, int H, int x,
                                          int y) {
   if (x < 0 || (unsigned)x >= (unsigned)W || y < 0 || (unsigned)y >= (unsigned)H) return false;
-  return glb(occ)[(size_t)x * (size_t)H + (size_t)y] != 0;
+  return glb(occ)[__umul24((unsigned)x, (unsigned)H) + (unsigned)y] != 0;  // W, H <= 65535: 32-bit
 }
 
 // BresenhamsLine::calc_range (RangeLib.h:696-769).  (s, c) = (sinf h, cosf h).
@@ -681,7 +681,7 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
     if (0.0f <= _y && _y < limy && 0.0f <= _x && _x < limx) {
       int cx = steep ? (int)_x : (int)_y;
       int cy = steep ? (int)_y : (int)_x;
-      if (glb(occ)[(size_t)cx * (size_t)H + (size_t)cy]) {
+      if (glb(occ)[__umul24((unsigned)cx, (unsigned)H) + (unsigned)cy]) {  // W, H <= 65535: 32-bit
         float xd = _x - x0;
         float yd = _y - y0;
         if (COUNT) *visits = nv;
