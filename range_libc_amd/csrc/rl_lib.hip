@@ -1083,7 +1083,8 @@ int build_edt(rl_map* m, unsigned flags) {
     // Pass 2: the bounded search where it is exact, the lower envelope on the
     // rows it marks (or on every row when the envelope is asked for).
     const int* rows = nullptr;
-    if (W <= 4096 && !(flags & RL_MAP_EDT_ENVELOPE_ONLY)) {
+    // the row search needs EDT_RB rows of the map in LDS (plus its flags) within 64 KiB
+    if ((size_t)EDT_RB * W * sizeof(float) + 64 <= 65536 && !(flags & RL_MAP_EDT_ENVELOPE_ONLY)) {
       int* slow = v + N;
       hipLaunchKernelGGL(k_edt_rows, dim3((H + EDT_RB - 1) / EDT_RB), dim3(256),
                          (size_t)EDT_RB * W * sizeof(float), c->stream, (const float*)g,

@@ -82,7 +82,7 @@ def _edt_gpu(ctx, img, flags=0):
 
 @pytest.mark.parametrize("case", ["rect512", "corridor1024", "empty", "full", "single_free",
                                   "one_px", "tall", "wide", "sparse_cols", "very_tall", "mixed_rows",
-                                  "wider_than_4096"])
+                                  "wider_than_4096", "width_4092", "width_4096"])
 def test_edt_bit_exact(ctx, case):
     rng = np.random.default_rng(7)
     img = {
@@ -99,6 +99,9 @@ def test_edt_bit_exact(ctx, case):
         # rows y >= 926 have g + x^2 >= 2^24 at x = 3990: the row search hands them to the envelope
         "mixed_rows": lambda: _points((1500, 4000), [(0, 3990), (700, 10)]),
         "wider_than_4096": lambda: (rng.random((5, 4200)) < 0.002).astype(np.uint8),  # envelope on every row
+        # the widest rows the LDS row search takes, and just past it (the envelope)
+        "width_4092": lambda: (rng.random((30, 4092)) < 0.003).astype(np.uint8),
+        "width_4096": lambda: (rng.random((30, 4096)) < 0.003).astype(np.uint8),
     }[case]()
     want = O.edt(img)
     assert_bits_equal(_edt_gpu(ctx, img), want, f"edt {case}")
