@@ -79,8 +79,19 @@ def cpu_baseline(img, ins, angles, obs, budget_s=10.0):
         if el >= budget_s:
             break
     rays = n * PARTICLES * BEAMS
-    return {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": kind,
-            "sample": f"{n} full steps ({PARTICLES}x{BEAMS} rays + sensor model) in {el:.1f} s"}
+    out = {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": kind,
+           "sample": f"{n} full steps ({PARTICLES}x{BEAMS} rays + sensor model) in {el:.1f} s"}
+    if kind == "reference" and threads > 1:  # the reference's own single-threaded path as well
+        n1, t0 = 0, time.perf_counter()
+        while True:
+            m.pf_step(ins, angles, obs, 1)
+            n1 += 1
+            el1 = time.perf_counter() - t0
+            if el1 >= budget_s / 2:
+                break
+        out["single_thread"] = {"value": n1 * PARTICLES * BEAMS / el1, "unit": "rays/s", "cores": 1,
+                                "sample": f"{n1} full steps in {el1:.1f} s"}
+    return out
 
 
 def bench_uneven64(args):
