@@ -283,6 +283,9 @@ int rl_debug_sqrtf_dev(rl_ctx* ctx, const float* d_x, float* d_out, int n);
 /* C = A * B through the surrogate's bf16 matrix-core GEMM (host row-major
  * matrices of bf16 bit patterns; fp32 result). */
 int rl_debug_gemm_bf16(rl_ctx* ctx, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K);
+/* rl_surrogate_forward through the per-layer kernels that training uses
+ * (rl_surrogate_forward itself runs one fused kernel with the same outputs). */
+int rl_debug_surrogate_forward_layers(rl_surrogate* s, const float* rays4, int n, float* out);
 /* C = A (M x K) * Bt (N x K)^T, the surrogate's forward-layer layout (the
  * 128 x 128 tile kernel when K % 32 == 0 and N % 128 == 0). */
 int rl_debug_gemm_bf16_nt(rl_ctx* ctx, const uint16_t* A, const uint16_t* Bt, float* C, int M, int N, int K);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "rl_surrogate_get_params": [_vp, _vp, _vp],
     "rl_surrogate_set_params": [_vp, _vp, _vp],
     "rl_surrogate_forward": [_vp, _vp, C.c_int, _vp],
+    "rl_debug_surrogate_forward_layers": [_vp, _vp, C.c_int, _vp],
     "rl_surrogate_gradient": [_vp, C.c_float, _vp, _vp, C.c_int, _fp, _vp, _vp],
     "rl_surrogate_step": [_vp, C.c_float],
     "rl_surrogate_train": [_vp, C.c_float, C.c_longlong, C.c_int, C.c_float, C.c_uint, _vp],
@@ -475,6 +476,13 @@ class Surrogate:
         r = np.ascontiguousarray(rays4, np.float32).reshape(-1, 4)
         out = np.empty((r.shape[0], 2), np.float32)
         check(lib().rl_surrogate_forward(self.handle, ptr(r), r.shape[0], ptr(out)))
+        return out
+
+    def forward_layers(self, rays4: np.ndarray) -> np.ndarray:
+        """forward() through the per-layer kernels that training uses (test hook)."""
+        r = np.ascontiguousarray(rays4, np.float32).reshape(-1, 4)
+        out = np.empty((r.shape[0], 2), np.float32)
+        check(lib().rl_debug_surrogate_forward_layers(self.handle, ptr(r), r.shape[0], ptr(out)))
         return out
 
     def gradient(self, max_range: float, rays4: np.ndarray, targets: np.ndarray):

@@ -10,7 +10,8 @@
 //   a1 = relu(W1 phi + b1), a2 = relu(W2 a1 + b2), a3 = relu(W3 a2 + b3)   (256 wide)
 //   o  = W4 a3 + b4,  range = R * o[0],  error bound = exp(o[1]) cells
 // Weights are stored in fp32 (master copy, Adam) and bf16 (MFMA operands);
-// products accumulate in fp32 (v_mfma_f32_16x16x16_bf16).
+// products accumulate in fp32 (v_mfma_f32_16x16x32_bf16 for the 256-wide
+// layers, v_mfma_f32_16x16x16_bf16 elsewhere).
 //
 // Loss per ray: Huber (delta 1 cell) on range - marched range, plus the
 // pinball loss at quantile 0.95 between the bound and the observed absolute
@@ -245,6 +246,110 @@ __global__ void k_sg_features(const float4* __restrict__ rays, const float* __re
                                     freq[4 * j + 3] * v3);
     out[(size_t)i * RL_SG_F + j] = f2bf(sinf(p));
     out[(size_t)i * RL_SG_F + RL_SG_NF + j] = f2bf(cosf(p));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: features, the three hidden layers and the heads of 64 rays
+// per 256-thread workgroup, the activations kept in LDS (one 64 x 256 bf16
+// tile, rewritten in place by each layer).  The outputs are bitwise those of
+// k_sg_features followed by the per-layer GEMMs: the same feature expression,
+// the same matrix instructions over k in the same ascending order from zero
+// accumulators (16x16x32 for the hidden layers as in k_gemm_bf16_nt, 16x16x16
+// for the heads as in k_gemm_bf16), the same bias / relu / bf16 epilogue.
+// Each row's outputs depend on that row alone, so the rows past n (zero
+// features) are computed and never stored.
+#define RL_SG_TM 64
+#define RL_SG_ACT_PAD (RL_SG_H + 8)  // row stride in elements: 16-byte reads of 16 rows spread over the banks
+
+// One hidden layer: act (64 x K) -> act (64 x 256), wave w computing columns
+// 64w .. 64w + 63.  W is the bf16 layer matrix [256][K] (K-contiguous).
+template <int K>
+__device__ __forceinline__ void sg_hidden_layer(bf16_t* act, const bf16_t* __restrict__ W,
+                                                const float* __restrict__ bias, int wave, int lane) {
+  const int kq = (lane >> 4) * 8, r16 = lane & 15;
+  const int nw = wave * 64;
+  sg_float4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = sg_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    sg_bf16x8 a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = *(const sg_bf16x8*)(W + (size_t)(nw + j * 16 + r16) * K + k0 + kq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *(const sg_bf16x8*)(act + (i * 16 + r16) * RL_SG_ACT_PAD + k0 + kq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  __syncthreads();  // every wave has read the layer's input before it is overwritten
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = nw + j * 16 + r16;
+    const float bn = bias[n];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float v = acc[i][j][q] + bn;
+        v = v < 0.0f ? 0.0f : v;
+        act[(i * 16 + 4 * (lane >> 4) + q) * RL_SG_ACT_PAD + n] = f2bf(v);
+      }
+  }
+  __syncthreads();
+}
+
+// rays: n x (x0, y0, sin h, cos h); P / Pf: the bf16 / fp32 parameter vectors;
+// out: n x 2 fp32.  Launched with ceil(n / 64) workgroups of 256 threads.
+__global__ __launch_bounds__(256) void k_sg_infer(const float4* __restrict__ rays, const float* __restrict__ freq,
+                                                  float inv_w, float inv_h, const bf16_t* __restrict__ P,
+                                                  const float* __restrict__ Pf, float* __restrict__ out, int n) {
+  __shared__ __align__(16) bf16_t act[RL_SG_TM * RL_SG_ACT_PAD];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int r0 = blockIdx.x * RL_SG_TM;
+  for (int e = tid; e < RL_SG_TM * RL_SG_NF; e += 256) {
+    const int r = e / RL_SG_NF, j = e - r * RL_SG_NF;
+    bf16_t s = 0, c = 0;
+    if (r0 + r < n) {
+      const float4 rr = rays[r0 + r];
+      const float v0 = rr.x * inv_w, v1 = rr.y * inv_h, v2 = rr.z, v3 = rr.w;
+      const float p = 6.283185307179586f * (freq[4 * j] * v0 + freq[4 * j + 1] * v1 + freq[4 * j + 2] * v2 +
+                                            freq[4 * j + 3] * v3);
+      s = f2bf(sinf(p));
+      c = f2bf(cosf(p));
+    }
+    act[r * RL_SG_ACT_PAD + j] = s;
+    act[r * RL_SG_ACT_PAD + RL_SG_NF + j] = c;
+  }
+  __syncthreads();
+  sg_hidden_layer<RL_SG_F>(act, P + SgLayout::W1, Pf + SgLayout::B1, wave, lane);
+  sg_hidden_layer<RL_SG_H>(act, P + SgLayout::W2, Pf + SgLayout::B2, wave, lane);
+  sg_hidden_layer<RL_SG_H>(act, P + SgLayout::W3, Pf + SgLayout::B3, wave, lane);
+  // heads: wave w takes rows 16w .. 16w + 15; columns 0 and 1 of one 16-wide tile
+  const int col = lane & 15, kb = 4 * (lane >> 4);
+  const bf16_t* W4 = P + SgLayout::W4;
+  sg_float4 acc = sg_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k = 0; k < RL_SG_H; k += 16) {
+    sg_short4 a, b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = (short)act[(wave * 16 + col) * RL_SG_ACT_PAD + k + kb + q];
+      b[q] = col < RL_SG_OUT ? (short)W4[col * RL_SG_H + k + kb + q] : (short)0;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  if (col < RL_SG_OUT) {
+    const float bn = Pf[SgLayout::B4 + col];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int m = r0 + wave * 16 + kb + q;
+      if (m < n) out[(size_t)m * RL_SG_OUT + col] = acc[q] + bn;
+    }
   }
 }
 
@@ -590,6 +695,17 @@ int sg_forward(rl_surrogate* s, int n) {
   return RL_OK;
 }
 
+// Network outputs of n rays already in d_rays into d_out, one fused launch
+// (no intermediate activations in HBM); bitwise equal to sg_forward's d_out.
+int sg_infer(rl_surrogate* s, int n) {
+  rl_map* m = s->map;
+  hipLaunchKernelGGL(k_sg_infer, dim3((unsigned)((n + RL_SG_TM - 1) / RL_SG_TM)), dim3(256), 0, m->ctx->stream,
+                     (const float4*)s->d_rays, (const float*)s->d_freq, 1.0f / (float)m->width,
+                     1.0f / (float)m->height, (const bf16_t*)s->d_pb, (const float*)s->d_p, s->d_out, n);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
 // Loss, gradient into d_g (fp32, layout of the parameters) for n rays with
 // targets in d_target.  grad_out (optional) receives d loss / d out (n x 2).
 int sg_backward(rl_surrogate* s, int n, float max_range, float* grad_out) {
@@ -798,7 +914,7 @@ int rl_surrogate_set_params(rl_surrogate* s, const float* params, const float* f
 }
 
 // Network outputs (n x 2) for n rays (x0, y0, sin h, cos h); host pointers.
-int rl_surrogate_forward(rl_surrogate* s, const float* rays4, int n, float* out) {
+static int surrogate_forward(rl_surrogate* s, const float* rays4, int n, float* out, bool layers) {
   CHECK_ARG(s && rays4 && out && n >= 0, "bad argument");
   std::lock_guard<std::mutex> lk(s->map->ctx->mu);
   TRY(use_device(s->map->ctx));
@@ -806,10 +922,14 @@ int rl_surrogate_forward(rl_surrogate* s, const float* rays4, int n, float* out)
   TRY(sg_reserve(s, n));
   hipStream_t st = s->map->ctx->stream;
   HIP_TRY(hipMemcpyAsync(s->d_rays, rays4, (size_t)n * 16, hipMemcpyHostToDevice, st));
-  TRY(sg_forward(s, n));
+  TRY(layers ? sg_forward(s, n) : sg_infer(s, n));
   HIP_TRY(hipMemcpyAsync(out, s->d_out, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return RL_OK;
+}
+
+int rl_surrogate_forward(rl_surrogate* s, const float* rays4, int n, float* out) {
+  return surrogate_forward(s, rays4, n, out, false);
 }
 
 // Loss and gradient for n rays with target ranges (host pointers); the bf16
@@ -909,7 +1029,7 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
   hipLaunchKernelGGL(k_sg_rays_from_particles, dim3(grid_for(n, 256)), dim3(256), 0, st, md, c->trig, d_ins,
                      d_angles, N, M, s->d_rays);
   HIP_TRY(hipGetLastError());
-  TRY(sg_forward(s, n));
+  TRY(sg_infer(s, n));
   hipLaunchKernelGGL(k_sg_checked_march, dim3(grid_for(n, 256)), dim3(256), 0, st, md, max_range,
                      (const float4*)s->d_rays, (const float*)s->d_out, bound_max, d_ranges, md.world.scale,
                      s->d_stats, n);
@@ -925,6 +1045,12 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
 // row-major host matrices, fp32 result.
 int rl_debug_gemm_bf16(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K) {
   return debug_gemm(c, A, B, C, M, N, K, false);
+}
+
+// rl_surrogate_forward through the per-layer kernels the training uses (test
+// hook for the fused inference kernel).
+int rl_debug_surrogate_forward_layers(rl_surrogate* s, const float* rays4, int n, float* out) {
+  return surrogate_forward(s, rays4, n, out, true);
 }
 
 // C = A(M x K) * Bt(N x K)^T: the forward-layer layout (K-contiguous operands).

@@ -112,6 +112,28 @@ def test_features_loss_gradient_vs_cpu(ctx, n):
     print("\n".join(report))
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 777, 3000])
+def test_fused_forward_equals_layers_and_cpu(ctx, n):
+    """forward() (one fused kernel: features, hidden layers and heads with the
+    activations in LDS) is bitwise the per-layer kernels' output, including
+    ragged last tiles, and matches the float64 network on the same features."""
+    img, g, rays = _map_and_rays(n, seed=5)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=9)
+    p, _ = sg.params()
+    p = p.copy()
+    rng = np.random.default_rng(n)
+    p[S.B4:S.B4 + 2] = rng.uniform(-1.0, 1.0, 2)  # non-zero head biases
+    sg.set_params(p)
+    fused = sg.forward(rays)
+    layers = sg.forward_layers(rays)
+    assert_bits_equal(fused, layers, "fused vs per-layer forward")
+    _, _, feat = sg.gradient(100.0, rays, np.zeros(n, np.float32))
+    _, want = S.forward(p, S.bf16_to_f64(feat), emulate_bf16=True)
+    for k in range(2):
+        assert _rel(fused[:, k].astype(np.float64), want[:, k]) <= 1e-2, k
+
+
 def test_adam_steps_vs_cpu(ctx):
     R = 120.0
     n = 2048
