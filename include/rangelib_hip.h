@@ -286,6 +286,9 @@ int rl_debug_gemm_bf16(rl_ctx* ctx, const uint16_t* A, const uint16_t* B, float*
 /* rl_surrogate_forward through the per-layer kernels that training uses
  * (rl_surrogate_forward itself runs one fused kernel with the same outputs). */
 int rl_debug_surrogate_forward_layers(rl_surrogate* s, const float* rays4, int n, float* out);
+/* Number of float bit patterns (of all 2^32) on which the fused surrogate
+ * kernel's bf16 conversion differs from the per-layer kernels' (expected 0). */
+int rl_debug_bf16_check(rl_ctx* ctx, unsigned long long* mismatches);
 /* C = A (M x K) * Bt (N x K)^T, the surrogate's forward-layer layout (the
  * 128 x 128 tile kernel when K % 32 == 0 and N % 128 == 0). */
 int rl_debug_gemm_bf16_nt(rl_ctx* ctx, const uint16_t* A, const uint16_t* Bt, float* C, int M, int N, int K);

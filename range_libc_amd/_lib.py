@@ -90,6 +90,7 @@ SIGNATURES = {
     "rl_surrogate_set_params": [_vp, _vp, _vp],
     "rl_surrogate_forward": [_vp, _vp, C.c_int, _vp],
     "rl_debug_surrogate_forward_layers": [_vp, _vp, C.c_int, _vp],
+    "rl_debug_bf16_check": [_vp, _vp],
     "rl_surrogate_gradient": [_vp, C.c_float, _vp, _vp, C.c_int, _fp, _vp, _vp],
     "rl_surrogate_step": [_vp, C.c_float],
     "rl_surrogate_train": [_vp, C.c_float, C.c_longlong, C.c_int, C.c_float, C.c_uint, _vp],
@@ -417,6 +418,14 @@ def debug_gemm_bf16(ctx: Context, a_bits: np.ndarray, b_bits: np.ndarray) -> np.
     c = np.empty((M, N), dtype=np.float32)
     check(lib().rl_debug_gemm_bf16(ctx.handle, ptr(a), ptr(b), ptr(c), M, N, K))
     return c
+
+
+def debug_bf16_check(ctx: Context) -> int:
+    """Float bit patterns (of all 2^32) where the fused surrogate kernel's bf16
+    conversion differs from the per-layer kernels'."""
+    n = C.c_ulonglong()
+    check(lib().rl_debug_bf16_check(ctx.handle, C.byref(n)))
+    return int(n.value)
 
 
 def debug_gemm_bf16_nt(ctx: Context, a_bits: np.ndarray, bt_bits: np.ndarray) -> np.ndarray:

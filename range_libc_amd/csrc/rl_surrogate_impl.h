@@ -250,54 +250,84 @@ __global__ void k_sg_features(const float4* __restrict__ rays, const float* __re
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: features, the three hidden layers and the heads of 64 rays
-// per 256-thread workgroup, the activations kept in LDS (one 64 x 256 bf16
-// tile, rewritten in place by each layer).  The outputs are bitwise those of
-// k_sg_features followed by the per-layer GEMMs: the same feature expression,
-// the same matrix instructions over k in the same ascending order from zero
-// accumulators (16x16x32 for the hidden layers as in k_gemm_bf16_nt, 16x16x16
-// for the heads as in k_gemm_bf16), the same bias / relu / bf16 epilogue.
-// Each row's outputs depend on that row alone, so the rows past n (zero
-// features) are computed and never stored.
+// Fused inference: features, the three hidden layers and the heads of 64
+// rays per 256-thread workgroup, the activations kept in LDS (one 64 x 256
+// bf16 tile, rewritten in place by each layer).  The outputs are bitwise those
+// of k_sg_features followed by the per-layer GEMMs: the same feature
+// expression, the same matrix instructions over k in the same ascending order
+// from zero accumulators (16x16x32 for the hidden layers as in
+// k_gemm_bf16_nt, 16x16x16 for the heads as in k_gemm_bf16), the same bias /
+// relu / bf16 epilogue.  Each row's outputs depend on that row alone, so the
+// rows past n (zero features) are computed and never stored.
 #define RL_SG_TM 64
 #define RL_SG_ACT_PAD (RL_SG_H + 8)  // row stride in elements: 16-byte reads of 16 rows spread over the banks
 
+// f2bf through the hardware's round-to-nearest-even conversion, NaN mapped to
+// 0x7fc0 as f2bf does: equal to f2bf for every float (all 2^32 bit patterns
+// compared by rl_debug_bf16_check).
+__device__ __forceinline__ bf16_t f2bf_hw(float v) {
+  const bf16_t r = __builtin_bit_cast(bf16_t, (__bf16)v);
+  return v != v ? (bf16_t)0x7fc0 : r;
+}
+
+__global__ void k_sg_bf16_check(unsigned long long* __restrict__ bad) {
+  unsigned long long local = 0;
+  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long u = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; u < (1ull << 32);
+       u += stride) {
+    const float f = __uint_as_float((unsigned)u);
+    local += f2bf_hw(f) != f2bf(f) ? 1u : 0u;
+  }
+  if (local) atomicAdd(bad, local);
+}
+
 // One hidden layer: act (64 x K) -> act (64 x 256), wave w computing columns
-// 64w .. 64w + 63.  W is the bf16 layer matrix [256][K] (K-contiguous).
-template <int K>
-__device__ __forceinline__ void sg_hidden_layer(bf16_t* act, const bf16_t* __restrict__ W,
-                                                const float* __restrict__ bias, int wave, int lane) {
+// 64w .. 64w + 63.  W is the bf16 layer matrix [256][K] (K-contiguous), sb the
+// layer's biases in LDS.  b0 holds the wave's fragments of the first k step,
+// loaded before the layer starts; the layer loads those of the next layer (Wn,
+// KN columns; KN = 0: none) before its own epilogue, so that no layer starts
+// with an L2 round trip.
+template <int K, int KN>
+__device__ __forceinline__ void sg_hidden_layer(bf16_t* act, const bf16_t* __restrict__ W, const float* sb,
+                                                const bf16_t* __restrict__ Wn, sg_bf16x8 (&b0)[4], int wave,
+                                                int lane) {
+  constexpr int RT = RL_SG_TM / 16, KS = K / 32;
   const int kq = (lane >> 4) * 8, r16 = lane & 15;
   const int nw = wave * 64;
-  sg_float4 acc[4][4];
+  sg_float4 acc[RT][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RT; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = sg_float4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    sg_bf16x8 a[4], b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *(const sg_bf16x8*)(W + (size_t)(nw + j * 16 + r16) * K + k0 + kq);
+  for (int s = 0; s < KS; ++s) {
+    sg_bf16x8 b[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = *(const sg_bf16x8*)(act + (i * 16 + r16) * RL_SG_ACT_PAD + k0 + kq);
+    for (int j = 0; j < 4; ++j)
+      b[j] = s == 0 ? b0[j] : *(const sg_bf16x8*)(W + (size_t)(nw + j * 16 + r16) * K + s * 32 + kq);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RT; ++i) {
+      const sg_bf16x8 a = *(const sg_bf16x8*)(act + (i * 16 + r16) * RL_SG_ACT_PAD + s * 32 + kq);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  if (KN) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b0[j] = *(const sg_bf16x8*)(Wn + (size_t)(nw + j * 16 + r16) * KN + kq);
   }
   __syncthreads();  // every wave has read the layer's input before it is overwritten
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int n = nw + j * 16 + r16;
-    const float bn = bias[n];
+    const float bb = sb[n];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        float v = acc[i][j][q] + bn;
+        float v = acc[i][j][q] + bb;
         v = v < 0.0f ? 0.0f : v;
-        act[(i * 16 + 4 * (lane >> 4) + q) * RL_SG_ACT_PAD + n] = f2bf(v);
+        act[(i * 16 + 4 * (lane >> 4) + q) * RL_SG_ACT_PAD + n] = f2bf_hw(v);
       }
   }
   __syncthreads();
@@ -305,12 +335,27 @@ __device__ __forceinline__ void sg_hidden_layer(bf16_t* act, const bf16_t* __res
 
 // rays: n x (x0, y0, sin h, cos h); P / Pf: the bf16 / fp32 parameter vectors;
 // out: n x 2 fp32.  Launched with ceil(n / 64) workgroups of 256 threads.
-__global__ __launch_bounds__(256) void k_sg_infer(const float4* __restrict__ rays, const float* __restrict__ freq,
+// Register use is held to 128 so that four waves share each SIMD (measured on
+// 240,000 rays: 3 waves, 144 registers, 5% slower).
+__global__ __launch_bounds__(256, 4) void k_sg_infer(const float4* __restrict__ rays, const float* __restrict__ freq,
                                                   float inv_w, float inv_h, const bf16_t* __restrict__ P,
                                                   const float* __restrict__ Pf, float* __restrict__ out, int n) {
   __shared__ __align__(16) bf16_t act[RL_SG_TM * RL_SG_ACT_PAD];
+  __shared__ float sbias[3 * RL_SG_H + RL_SG_OUT];         // b1, b2, b3, b4
+  __shared__ __align__(16) bf16_t sw4[RL_SG_OUT * RL_SG_H];  // W4
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int r0 = blockIdx.x * RL_SG_TM;
+  sg_bf16x8 b0[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    b0[j] = *(const sg_bf16x8*)(P + SgLayout::W1 + (size_t)(wave * 64 + j * 16 + (lane & 15)) * RL_SG_F +
+                                (lane >> 4) * 8);
+  sbias[tid] = Pf[SgLayout::B1 + tid];
+  sbias[RL_SG_H + tid] = Pf[SgLayout::B2 + tid];
+  sbias[2 * RL_SG_H + tid] = Pf[SgLayout::B3 + tid];
+  if (tid < RL_SG_OUT) sbias[3 * RL_SG_H + tid] = Pf[SgLayout::B4 + tid];
+  sw4[tid] = P[SgLayout::W4 + tid];
+  sw4[RL_SG_H + tid] = P[SgLayout::W4 + RL_SG_H + tid];
   for (int e = tid; e < RL_SG_TM * RL_SG_NF; e += 256) {
     const int r = e / RL_SG_NF, j = e - r * RL_SG_NF;
     bf16_t s = 0, c = 0;
@@ -326,29 +371,29 @@ __global__ __launch_bounds__(256) void k_sg_infer(const float4* __restrict__ ray
     act[r * RL_SG_ACT_PAD + RL_SG_NF + j] = c;
   }
   __syncthreads();
-  sg_hidden_layer<RL_SG_F>(act, P + SgLayout::W1, Pf + SgLayout::B1, wave, lane);
-  sg_hidden_layer<RL_SG_H>(act, P + SgLayout::W2, Pf + SgLayout::B2, wave, lane);
-  sg_hidden_layer<RL_SG_H>(act, P + SgLayout::W3, Pf + SgLayout::B3, wave, lane);
-  // heads: wave w takes rows 16w .. 16w + 15; columns 0 and 1 of one 16-wide tile
+  sg_hidden_layer<RL_SG_F, RL_SG_H>(act, P + SgLayout::W1, sbias, P + SgLayout::W2, b0, wave, lane);
+  sg_hidden_layer<RL_SG_H, RL_SG_H>(act, P + SgLayout::W2, sbias + RL_SG_H, P + SgLayout::W3, b0, wave, lane);
+  sg_hidden_layer<RL_SG_H, 0>(act, P + SgLayout::W3, sbias + 2 * RL_SG_H, nullptr, b0, wave, lane);
+  // heads: wave w takes rows 16w .. 16w + 15 (RL_SG_TM / 64 tiles of 16 rows
+  // per wave); columns 0 and 1 of one 16-wide tile
   const int col = lane & 15, kb = 4 * (lane >> 4);
-  const bf16_t* W4 = P + SgLayout::W4;
-  sg_float4 acc = sg_float4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int k = 0; k < RL_SG_H; k += 16) {
-    sg_short4 a, b;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = (short)act[(wave * 16 + col) * RL_SG_ACT_PAD + k + kb + q];
-      b[q] = col < RL_SG_OUT ? (short)W4[col * RL_SG_H + k + kb + q] : (short)0;
+  for (int t = 0; t < RL_SG_TM / 64; ++t) {
+    const int rt = (wave * (RL_SG_TM / 64) + t) * 16;
+    sg_float4 acc = sg_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < RL_SG_H; k += 16) {
+      const sg_short4 a = *(const sg_short4*)(act + (rt + col) * RL_SG_ACT_PAD + k + kb);
+      const sg_short4 b = col < RL_SG_OUT ? *(const sg_short4*)(sw4 + col * RL_SG_H + k + kb) : sg_short4{0, 0, 0, 0};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-  }
-  if (col < RL_SG_OUT) {
-    const float bn = Pf[SgLayout::B4 + col];
+    if (col < RL_SG_OUT) {
+      const float bb = sbias[3 * RL_SG_H + col];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int m = r0 + wave * 16 + kb + q;
-      if (m < n) out[(size_t)m * RL_SG_OUT + col] = acc[q] + bn;
+      for (int q = 0; q < 4; ++q) {
+        const int m = r0 + rt + kb + q;
+        if (m < n) out[(size_t)m * RL_SG_OUT + col] = acc[q] + bb;
+      }
     }
   }
 }
@@ -1051,6 +1096,26 @@ int rl_debug_gemm_bf16(rl_ctx* c, const uint16_t* A, const uint16_t* B, float* C
 // hook for the fused inference kernel).
 int rl_debug_surrogate_forward_layers(rl_surrogate* s, const float* rays4, int n, float* out) {
   return surrogate_forward(s, rays4, n, out, true);
+}
+
+// Mismatches between the fused kernel's bf16 conversion and f2bf over all
+// 2^32 float bit patterns (test hook).
+int rl_debug_bf16_check(rl_ctx* c, unsigned long long* mismatches) {
+  CHECK_ARG(c && mismatches, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  unsigned long long* d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(*d)));
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(*d), c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sg_bf16_check, dim3(4096), dim3(256), 0, c->stream, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, sizeof(*d), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  HIP_TRY(e);
+  return RL_OK;
 }
 
 // C = A(M x K) * Bt(N x K)^T: the forward-layer layout (K-contiguous operands).

@@ -112,6 +112,12 @@ def test_features_loss_gradient_vs_cpu(ctx, n):
     print("\n".join(report))
 
 
+def test_fused_bf16_conversion_exhaustive(ctx):
+    """The fused kernel's hardware bf16 conversion equals the per-layer
+    kernels' software rounding on every float bit pattern."""
+    assert rl._lib.debug_bf16_check(ctx) == 0
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 777, 3000])
 def test_fused_forward_equals_layers_and_cpu(ctx, n):
     """forward() (one fused kernel: features, hidden layers and heads with the
