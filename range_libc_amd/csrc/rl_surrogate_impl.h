@@ -104,9 +104,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = sg_float4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = kbeg; k0 < kend; k0 += 32) {
-    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread;
+    // neighbouring threads take neighbouring k, or neighbouring m when A is
+    // stored m-contiguous (the weight-gradient GEMMs), so that reads coalesce
+    const bool a_mfast = g.sam == 1 && g.sak != 1;
     for (int e = tid; e < 64 * 32; e += 256) {
-      int r = e >> 5, c = e & 31;
+      int r = a_mfast ? (e & 63) : (e >> 5), c = a_mfast ? (e >> 6) : (e & 31);
       int m = m0 + r, k = k0 + c;
       As[r][c] = (m < g.M && k < kend) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (bf16_t)0;
       int kr = e >> 6, nc = e & 63;
@@ -443,7 +446,8 @@ __global__ void k_sg_colsum(const bf16_t* __restrict__ D, int rows, int cols, in
   if (c >= cols) return;
   int r0 = blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
   float s = 0.0f;
-  for (int r = r0; r < r1; ++r) s += bf2f(D[(size_t)r * cols + c]);
+#pragma unroll 8
+  for (int r = r0; r < r1; ++r) s += bf2f(D[(size_t)r * cols + c]);  // loads issued ahead, sums in order
   part[(size_t)blockIdx.y * cols + c] = s;
 }
 
@@ -452,7 +456,8 @@ __global__ void k_sg_reduce(const float* __restrict__ part, int slices, int coun
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   float s = 0.0f;
-  for (int z = 0; z < slices; ++z) s += part[(size_t)z * count + i];
+#pragma unroll 8
+  for (int z = 0; z < slices; ++z) s += part[(size_t)z * count + i];  // loads issued ahead, sums in order
   dst[i] = s;
 }
 
