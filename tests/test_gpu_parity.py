@@ -277,6 +277,29 @@ def test_bresenham_vs_rm_full_size(corridor_map):
         assert_bits_equal(got[pick], O.Oracle(corridor_map, R, kind).numpy_calc_range(q[pick]), f"{kind} sample")
 
 
+@pytest.mark.parametrize("R", [1000.0, 40.0, 31.0])
+def test_bresenham_all_rays(corridor_map, R):
+    """Every one of 100,000 Bresenham rays on the 2048^2 corridor map equals the
+    oracle, in the grid and the world frame, for ranges whose walks end within
+    about 32 steps or run long, including origins off the map."""
+    torch = _torch()
+    ctx = rl.default_context(0)
+    n = 100_000
+    q = W.world_poses(corridor_map, n, seed=21)
+    rng = np.random.default_rng(22)
+    q[:500, 0] = rng.uniform(-60.0, 2108.0, 500)  # some origins off the map
+    q[:500, 1] = rng.uniform(-60.0, 2108.0, 500)
+    m = rl.DeviceMap(ctx, O.xmajor(corridor_map))
+    d_q = torch.from_numpy(q).cuda()
+    orc = O.Oracle(corridor_map, R, "bl")
+    for name, call, want in (("world", m.numpy_calc_range_dev, orc.numpy_calc_range(q)),
+                             ("grid", m.calc_range_many_dev, orc.calc_range_many(q))):
+        out = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+        call(rl.RL_METHOD_BL, R, d_q.data_ptr(), out.data_ptr(), n)
+        ctx.sync()
+        assert_bits_equal(out.cpu().numpy(), want, f"{name} R={R}")
+
+
 # --------------------------------------------------------------------------- step counts
 
 def test_step_counts_match_oracle(corridor_map):
