@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   for (int k0 = kbeg; k0 < kend; k0 += 32) {
     // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread.
     // (Taking neighbouring m per thread when A is m-contiguous, as in the
-    // weight-gradient GEMMs, made training 6% slower: 0.967 vs 0.910 s.)
+    // weight-gradient GEMMs, made training 6% slower: 0.967 vs 0.910 s; 4%
+    // slower when limited to the 256-row ones.)
     for (int e = tid; e < 64 * 32; e += 256) {
       int r = e >> 5, c = e & 31;
       int m = m0 + r, k = k0 + c;
