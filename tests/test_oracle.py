@@ -61,6 +61,32 @@ def test_png_decode_matches_reference(key):
         assert np.array_equal(got, want), (key, thr)
 
 
+def test_ros_occupancy_grid_ingestion():
+    """PyOMap(OccupancyGrid) as RangeLibc.pyx:146-167 builds it: OMap(height,
+    width) with grid[x][y] = data[x * width + y] > 10 (x over the message's
+    rows; -1 unknown and 0..10 free), and the world frame from the map origin,
+    angle = -yaw of its quaternion."""
+    from types import SimpleNamespace as NS
+
+    import range_libc_amd as rl
+
+    h, w, yaw = 5, 7, 0.4
+    data = np.random.default_rng(4).choice([-1, 0, 5, 10, 11, 50, 100], h * w)
+    quat = NS(x=0.0, y=0.0, z=math.sin(yaw / 2), w=math.cos(yaw / 2))
+    msg = NS(info=NS(width=w, height=h, resolution=0.05,
+                     origin=NS(position=NS(x=1.5, y=-2.0, z=0.0), orientation=quat)),
+             data=[int(v) for v in data])
+    m = rl.PyOMap(msg)
+    grid = data.reshape(h, w)
+    assert (m.width(), m.height()) == (h, w)
+    for x in range(h):
+        for y in range(w):
+            assert m.isOccupied(x, y) == bool(grid[x, y] > 10), (x, y)
+    scale, angle, ox, oy, s, c = m.world
+    assert (scale, ox, oy) == (0.05, 1.5, -2.0)
+    assert abs(angle + yaw) < 1e-12 and abs(s - math.sin(-yaw)) < 1e-12 and abs(c - math.cos(-yaw)) < 1e-12
+
+
 # --------------------------------------------------------------------------- distance transform
 
 @pytest.mark.parametrize("key", PNG_KEYS)
