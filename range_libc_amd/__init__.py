@@ -120,11 +120,13 @@ class PyOMap:
         self.world = (float(scale), float(angle), float(origin_x), float(origin_y), float(s), float(c))
 
     def save(self, fn) -> bool:
-        """White = free, black = occupied PNG (OMap::save, RangeLib.h:262-291)."""
+        """RGBA PNG, opaque, black where occupied and white elsewhere (OMap::save,
+        RangeLib.h:264-291); returns False on success as the reference's error code."""
         from PIL import Image
 
-        img = np.where(self.occ.T != 0, 0, 255).astype(np.uint8)
-        Image.fromarray(img, mode="L").save(fn)
+        v = np.where(self.occ.T != 0, 0, 255).astype(np.uint8)
+        rgba = np.stack([v, v, v, np.full_like(v, 255)], axis=-1)
+        Image.fromarray(rgba, mode="RGBA").save(fn)
         return False
 
     def isOccupied(self, x: int, y: int) -> bool:

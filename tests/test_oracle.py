@@ -61,6 +61,23 @@ def test_png_decode_matches_reference(key):
         assert np.array_equal(got, want), (key, thr)
 
 
+def test_save_round_trip(tmp_path):
+    """PyOMap.save writes OMap::save's image (RangeLib.h:264-291: opaque RGBA,
+    black where occupied, white elsewhere), which PyOMap(path) reads back."""
+    from PIL import Image
+
+    import range_libc_amd as rl
+
+    img = W.rect_disk_map(64, seed=3)[:, :50]  # non-square: 50 wide, 64 high
+    m = rl.PyOMap(img)
+    path = str(tmp_path / "map.png")
+    assert m.save(path) is False
+    rgba = np.asarray(Image.open(path))
+    assert rgba.shape == (64, 50, 4) and np.all(rgba[..., 3] == 255)
+    assert np.array_equal(rgba[..., 0] == 0, img != 0) and np.array_equal(rgba[..., :3].min(-1), rgba[..., :3].max(-1))
+    assert np.array_equal(rl.PyOMap(path).occ, m.occ)
+
+
 def test_ros_occupancy_grid_ingestion():
     """PyOMap(OccupancyGrid) as RangeLibc.pyx:146-167 builds it: OMap(height,
     width) with grid[x][y] = data[x * width + y] > 10 (x over the message's
