@@ -78,6 +78,21 @@ def test_save_round_trip(tmp_path):
     assert np.array_equal(rl.PyOMap(path).occ, m.occ)
 
 
+def test_png_load_failure_flags_error(tmp_path, capsys):
+    """OMap(filename) on an unreadable file prints the decoder error and
+    reports error() (RangeLib.h:159-168); the map left behind is 1x1 and free
+    here (the reference's grid is empty)."""
+    import range_libc_amd as rl
+
+    bad = tmp_path / "not_a.png"
+    bad.write_bytes(b"not a png")
+    for path in (str(bad), str(tmp_path / "missing.png")):
+        m = rl.PyOMap(path)
+        assert m.error() and (m.width(), m.height()) == (1, 1)
+    assert "ERROR" in capsys.readouterr().out
+    assert not rl.PyOMap(np.zeros((3, 4), np.uint8)).error()
+
+
 def test_ros_occupancy_grid_ingestion():
     """PyOMap(OccupancyGrid) as RangeLibc.pyx:146-167 builds it: OMap(height,
     width) with grid[x][y] = data[x * width + y] > 10 (x over the message's
