@@ -40,6 +40,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
                                                TrigDev trig, int use_box) {
   __shared__ float lut[RL_LUT_SIZE];
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  __shared__ double terms[4 * 64];
   if (METHOD == RL_METHOD_RM) {  // one distance table per context: the same for every map
     for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = maps[0].m.steps[i];
     __syncthreads();
@@ -60,11 +61,11 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
     if (bm.sensor)
       particle_beams<METHOD, true>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
                                    bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                   boxes + wib * RL_BOX_BYTES, none);
+                                   boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
     else
       particle_beams<METHOD, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
                                     bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                    boxes + wib * RL_BOX_BYTES, none);
+                                    boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
   }
 }
 

@@ -485,6 +485,32 @@ __device__ __forceinline__ BeamIn load_beam_in(const float* __restrict__ ins, co
 // range is read back as the separate call would (range * inv_world_scale), so
 // both outputs are bit-identical to the two calls.  Must be reached by the
 // whole wave (the product broadcasts across lanes).
+// segment_product for one particle per wave (S = 64), through LDS: lane j
+// stores its term in slot[j], then every lane reads the terms back in beam
+// order (one address per read, broadcast to the wave) and multiplies them into
+// w in that order, so the result is the same bits.  The terms stay in vector
+// registers; the lane reads of segment_product go through scalar registers.
+__device__ __forceinline__ double wave_product_lds(double w, double term, int cnt, double* slot) {
+  RL_LDS double* t = (RL_LDS double*)slot;
+  // the previous chunk's reads of the slot come before this chunk's writes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  t[threadIdx.x & 63] = term;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int jj = 0;
+  for (; jj + 8 <= cnt; jj += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = t[jj + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w *= v[q];
+  }
+  for (; jj < cnt; ++jj) w *= t[jj];
+  return w;
+}
+
 template <int METHOD, bool SENSOR>
 __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut, const TrigDev& trig,
                                                float max_range, const float* __restrict__ ins,
@@ -493,7 +519,8 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
                                                const double* __restrict__ table, int K,
                                                float* __restrict__ ranges,
                                                double* __restrict__ weights, int p, int M, int S,
-                                               int j0, int base, uint8_t* box_lds, BeamIn pre) {
+                                               int j0, int base, uint8_t* box_lds, double* terms_lds,
+                                               BeamIn pre) {
   const bool valid = p >= 0;
   const float hi = (float)((double)(float)K - 1.0);
   float gx = 0.0f, gy = 0.0f, theta = 0.0f;
@@ -525,7 +552,9 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
         term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)dd];
       }
     }
-    if (SENSOR) w = segment_product(w, term, base, min(S, M - c0), S);
+    if (SENSOR)
+      w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms_lds)
+                  : segment_product(w, term, base, min(S, M - c0), S);
   }
   if (SENSOR && valid && j0 == 0) weights[p] = w;
 }
@@ -548,8 +577,10 @@ __global__ __launch_bounds__(256) void k_particles(MapDev m, TrigDev trig, float
   __shared__ float lut[RL_LUT_SIZE];
   stage_lut<METHOD>(m, lut);
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  __shared__ double terms[4 * 64];
   particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights, p, M, S,
-                                 j0, base, boxes + (threadIdx.x >> 6) * RL_BOX_BYTES, pre);
+                                 j0, base, boxes + (threadIdx.x >> 6) * RL_BOX_BYTES, terms + (threadIdx.x >> 6) * 64,
+                                 pre);
 }
 
 // Particles of one region chunk per workgroup (see Sched).
@@ -577,8 +608,9 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
   __shared__ float lut[RL_LUT_SIZE];
   stage_lut<METHOD>(m, lut);
   __shared__ __align__(16) uint8_t boxes[4 * RL_BOX_BYTES];
+  __shared__ double terms[4 * 64];
   particle_beams<METHOD, SENSOR>(m, lut, trig, max_range, ins, angles, obs, table, K, ranges, weights, p, M, S,
-                                 j0, base, boxes + wib * RL_BOX_BYTES, pre);
+                                 j0, base, boxes + wib * RL_BOX_BYTES, terms + wib * 64, pre);
 }
 
 // GiantLUTCast's ray directions: (cosf, sinf) of angle i * step for i < td, the
