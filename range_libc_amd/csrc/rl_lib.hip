@@ -333,6 +333,32 @@ __device__ __forceinline__ double segment_product(double w, double term, int bas
   return w;
 }
 
+// segment_product for one particle per wave (S = 64), through LDS: lane j
+// stores its term in slot[j], then every lane reads the terms back in beam
+// order (one address per read, broadcast to the wave) and multiplies them into
+// w in that order, so the result is the same bits.  The terms stay in vector
+// registers; the lane reads of segment_product go through scalar registers.
+__device__ __forceinline__ double wave_product_lds(double w, double term, int cnt, double* slot) {
+  RL_LDS double* t = (RL_LDS double*)slot;
+  // the previous chunk's reads of the slot come before this chunk's writes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  t[threadIdx.x & 63] = term;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int jj = 0;
+  for (; jj + 8 <= cnt; jj += 8) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = t[jj + q];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w *= v[q];
+  }
+  for (; jj < cnt; ++jj) w *= t[jj];
+  return w;
+}
+
 __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
                                                 const float* __restrict__ ranges,
                                                 double* __restrict__ outs,
@@ -345,6 +371,7 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
   const int p = wave * per_wave + seg;
   const bool valid = p < N;
   const float hi = (float)((double)(float)K - 1.0);
+  __shared__ double terms[4 * 64];
   double w = 1.0;
   for (int c0 = 0; c0 < M; c0 += S) {
     int j = c0 + j0;
@@ -354,7 +381,8 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
       float d = rl::clamp_table(ranges[(size_t)p * M + j] * inv_scale, hi);
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = segment_product(w, term, base, min(S, M - c0), S);
+    w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
+                : segment_product(w, term, base, min(S, M - c0), S);
   }
   if (valid && j0 == 0) outs[p] = w;
 }
@@ -374,6 +402,7 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
   const bool valid = p < N;
   const float hi = (float)((double)(float)K - 1.0);
   __shared__ float lut[RL_LUT_SIZE];
+  __shared__ double terms[4 * 64];
   stage_lut<METHOD>(m, lut);
   float gx = 0.0f, gy = 0.0f, theta = 0.0f;
   if (valid) {  // RangeLib.h:589-597
@@ -393,7 +422,8 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
       float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = segment_product(w, term, base, min(S, M - c0), S);
+    w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
+                : segment_product(w, term, base, min(S, M - c0), S);
   }
   if (valid && j0 == 0) weights[p] = w;
 }
@@ -485,32 +515,6 @@ __device__ __forceinline__ BeamIn load_beam_in(const float* __restrict__ ins, co
 // range is read back as the separate call would (range * inv_world_scale), so
 // both outputs are bit-identical to the two calls.  Must be reached by the
 // whole wave (the product broadcasts across lanes).
-// segment_product for one particle per wave (S = 64), through LDS: lane j
-// stores its term in slot[j], then every lane reads the terms back in beam
-// order (one address per read, broadcast to the wave) and multiplies them into
-// w in that order, so the result is the same bits.  The terms stay in vector
-// registers; the lane reads of segment_product go through scalar registers.
-__device__ __forceinline__ double wave_product_lds(double w, double term, int cnt, double* slot) {
-  RL_LDS double* t = (RL_LDS double*)slot;
-  // the previous chunk's reads of the slot come before this chunk's writes
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  t[threadIdx.x & 63] = term;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int jj = 0;
-  for (; jj + 8 <= cnt; jj += 8) {
-    double v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = t[jj + q];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) w *= v[q];
-  }
-  for (; jj < cnt; ++jj) w *= t[jj];
-  return w;
-}
-
 template <int METHOD, bool SENSOR>
 __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut, const TrigDev& trig,
                                                float max_range, const float* __restrict__ ins,
