@@ -307,28 +307,9 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
 // <= 64); lane j of a segment handles beams j, j + S, ...  The weight is the
 // product over beams in beam order, as the reference's loop forms it
 // (RangeLib.h:542-554), so it is bit-identical: the terms are broadcast
-// across the segment with shuffles and multiplied sequentially.
-__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt, int S) {
-  if (S == 64) {  // one particle per wave: broadcast with scalar lane reads
-    int lo = __double2loint(term), hi = __double2hiint(term);
-    int jj = 0;
-    for (; jj + 4 <= cnt; jj += 4) {  // the lane reads of four terms ahead of their (ordered) products
-      const double t0 = __hiloint2double(__builtin_amdgcn_readlane(hi, jj), __builtin_amdgcn_readlane(lo, jj));
-      const double t1 =
-          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 1), __builtin_amdgcn_readlane(lo, jj + 1));
-      const double t2 =
-          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 2), __builtin_amdgcn_readlane(lo, jj + 2));
-      const double t3 =
-          __hiloint2double(__builtin_amdgcn_readlane(hi, jj + 3), __builtin_amdgcn_readlane(lo, jj + 3));
-      w *= t0;
-      w *= t1;
-      w *= t2;
-      w *= t3;
-    }
-    for (; jj < cnt; ++jj)
-      w *= __hiloint2double(__builtin_amdgcn_readlane(hi, jj), __builtin_amdgcn_readlane(lo, jj));
-    return w;
-  }
+// across the segment with shuffles and multiplied sequentially.  S < 64 here;
+// one particle per wave (S = 64) takes wave_product_lds.
+__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt) {
   for (int jj = 0; jj < cnt; ++jj) w *= __shfl(term, base + jj, 64);
   return w;
 }
@@ -337,7 +318,8 @@ __device__ __forceinline__ double segment_product(double w, double term, int bas
 // stores its term in slot[j], then every lane reads the terms back in beam
 // order (one address per read, broadcast to the wave) and multiplies them into
 // w in that order, so the result is the same bits.  The terms stay in vector
-// registers; the lane reads of segment_product go through scalar registers.
+// registers (reading them out lane by lane into scalar registers took 0.8 us
+// more per particle-filter step).
 __device__ __forceinline__ double wave_product_lds(double w, double term, int cnt, double* slot) {
   RL_LDS double* t = (RL_LDS double*)slot;
   // the previous chunk's reads of the slot come before this chunk's writes
@@ -382,7 +364,7 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
     w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
-                : segment_product(w, term, base, min(S, M - c0), S);
+                : segment_product(w, term, base, min(S, M - c0));
   }
   if (valid && j0 == 0) outs[p] = w;
 }
@@ -423,7 +405,7 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
     w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
-                : segment_product(w, term, base, min(S, M - c0), S);
+                : segment_product(w, term, base, min(S, M - c0));
   }
   if (valid && j0 == 0) weights[p] = w;
 }
@@ -558,7 +540,7 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
     }
     if (SENSOR)
       w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms_lds)
-                  : segment_product(w, term, base, min(S, M - c0), S);
+                  : segment_product(w, term, base, min(S, M - c0));
   }
   if (SENSOR && valid && j0 == 0) weights[p] = w;
 }
