@@ -197,7 +197,8 @@ def test_eval_sensor_model_bit_exact(m, corridor_map):
     assert np.array_equal(w, orc.eval_sensor_model(obs, ranges, m, n))
 
 
-@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("bl", 60)])
+@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("rm", 64), ("rm", 65), ("rm", 128),
+                                    ("bl", 60)])
 def test_fused_sensor_model_bit_exact(kind, m, corridor_map):
     table = W.sensor_table(501)
     ins = W.world_poses(corridor_map, 2000, seed=10)
@@ -380,7 +381,8 @@ def test_map_from_precomputed_edt():
         rl.DeviceMap(ctx, occ, edt_xmajor=edt[:-1])
 
 
-@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("bl", 60)])
+@pytest.mark.parametrize("kind,m", [("rm", 60), ("rm", 30), ("rm", 100), ("rm", 64), ("rm", 65), ("rm", 128),
+                                    ("bl", 60)])
 def test_pf_step_bit_exact(kind, m, corridor_map):
     """rl_pf_step_dev == numpy_calc_range_angles then eval_sensor_model."""
     torch = _torch()
