@@ -306,29 +306,14 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
 // Particles are laid out in wave segments of S lanes (S a power of two,
 // <= 64); lane j of a segment handles beams j, j + S, ...  The weight is the
 // product over beams in beam order, as the reference's loop forms it
-// (RangeLib.h:542-554), so it is bit-identical: the terms are broadcast
-// across the segment with shuffles and multiplied sequentially.  S < 64 here;
-// one particle per wave (S = 64) takes wave_product_lds.
-__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt) {
-  for (int jj = 0; jj < cnt; ++jj) w *= __shfl(term, base + jj, 64);
-  return w;
-}
-
-// segment_product for one particle per wave (S = 64), through LDS: lane j
-// stores its term in slot[j], then every lane reads the terms back in beam
-// order (one address per read, broadcast to the wave) and multiplies them into
-// w in that order, so the result is the same bits.  The terms stay in vector
-// registers (reading them out lane by lane into scalar registers took 0.8 us
-// more per particle-filter step).
-__device__ __forceinline__ double wave_product_lds(double w, double term, int cnt, double* slot) {
-  RL_LDS double* t = (RL_LDS double*)slot;
-  // the previous chunk's reads of the slot come before this chunk's writes
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  t[threadIdx.x & 63] = term;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// (RangeLib.h:542-554), so it is bit-identical: lane l stores its term in
+// slot[l] (a 64-double LDS slot per wave), then every lane reads its
+// segment's terms slot[base], slot[base + 1], ... back in beam order (one
+// address per segment and read) and multiplies them into w in that order.
+// The terms stay in vector registers (reading them out lane by lane into
+// scalar registers took 0.8 us more per particle-filter step, and cross-lane
+// shuffles put two permutes on the chain per term).
+__device__ __forceinline__ double slot_product(double w, const RL_LDS double* t, int cnt) {
   int jj = 0;
   for (; jj + 8 <= cnt; jj += 8) {
     double v[8];
@@ -339,6 +324,21 @@ __device__ __forceinline__ double wave_product_lds(double w, double term, int cn
   }
   for (; jj < cnt; ++jj) w *= t[jj];
   return w;
+}
+
+__device__ __forceinline__ double segment_product(double w, double term, int base, int cnt, int S, double* slot) {
+  RL_LDS double* t = (RL_LDS double*)slot;
+  // the previous chunk's reads of the slot come before this chunk's writes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  t[threadIdx.x & 63] = term;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // one particle per wave: the same addresses in every lane (a per-lane
+  // segment offset there cost 0.2 us per particle-filter step)
+  if (S == 64) return slot_product(w, t, cnt);
+  return slot_product(w, t + base, cnt);
 }
 
 __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
@@ -363,8 +363,7 @@ __global__ __launch_bounds__(256) void k_sensor(const float* __restrict__ obs,
       float d = rl::clamp_table(ranges[(size_t)p * M + j] * inv_scale, hi);
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
-                : segment_product(w, term, base, min(S, M - c0));
+    w = segment_product(w, term, base, min(S, M - c0), S, terms + (threadIdx.x >> 6) * 64);
   }
   if (valid && j0 == 0) outs[p] = w;
 }
@@ -404,8 +403,7 @@ __global__ __launch_bounds__(256) void k_fused(MapDev m, TrigDev trig, float max
       float r = rl::clamp_table(obs[j] * m.world.inv_scale, hi);
       term = rl::glb(table)[(size_t)(int)r * (size_t)K + (size_t)(int)d];
     }
-    w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms + (threadIdx.x >> 6) * 64)
-                : segment_product(w, term, base, min(S, M - c0));
+    w = segment_product(w, term, base, min(S, M - c0), S, terms + (threadIdx.x >> 6) * 64);
   }
   if (valid && j0 == 0) weights[p] = w;
 }
@@ -539,8 +537,7 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
       }
     }
     if (SENSOR)
-      w = S == 64 ? wave_product_lds(w, term, min(S, M - c0), terms_lds)
-                  : segment_product(w, term, base, min(S, M - c0));
+      w = segment_product(w, term, base, min(S, M - c0), S, terms_lds);
   }
   if (SENSOR && valid && j0 == 0) weights[p] = w;
 }
