@@ -4,15 +4,18 @@ Workload (default): 2048x2048 synthetic corridor map, 4000 particles in free
 space, ``calc_range_repeat_angles`` with 60 beams over 270 degrees and
 max_range 500 px (240,000 rays), then ``eval_sensor_model`` with a 501x501
 table against 60 observed ranges.  Inputs are device-resident when the timed
-region starts; a step is the two launches (ray marching over repeated angles,
-sensor model).
+region starts; a step is one fused launch (ray marching over repeated angles
+with the sensor-model product of each particle), bit-identical to the two
+calls, which are timed beside it.
 
-Prints one JSON line: rays/s, the ray-marching kernel's fraction of the HBM
-roofline (algorithmic bytes = 4 B per distance-transform read, counted per ray
-on the device, plus pose/angle input and range output), and the reference CPU
-path timed on this host.
+Prints one JSON line: rays/s, the fused kernel's fraction of the HBM roofline
+(algorithmic bytes = 4 B per distance-transform read, counted per ray on the
+device, plus pose/angle/observation input, range output, one sensor-table entry
+per ray and one weight per particle), the longest dependent chain of reads of
+the launch, and the reference CPU path timed on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --workload {c1,uneven64,surrogate,glt,cddt,bresenham}
 """
 from __future__ import annotations
 
@@ -46,6 +49,45 @@ def build_workload(seed_map=1, seed_pose=2, seed_true=3, seed_noise=4):
     return img, ins, angles, table, true_pose, seed_noise
 
 
+def host_info() -> dict:
+    """CPU model, usable cores and C library of the host the CPU leg runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        libc = os.confstr("CS_GNU_LIBC_VERSION")
+    except (ValueError, OSError):
+        libc = "unknown"
+    return {"cpu_model": model, "host_cores": len(os.sched_getaffinity(0)), "libc": libc}
+
+
+def timed_runs(fn, budget_s: float, min_runs: int = 10, warmup: int = 2):
+    """Wall times of repeated runs of fn: `warmup` untimed runs, then at least
+    `min_runs` timed ones and more while the budget lasts."""
+    for _ in range(warmup):
+        fn()
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < min_runs or time.perf_counter() < t_end:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def rate_entry(units_per_run: float, times, unit: str, cores: int, kind: str, sample: str) -> dict:
+    med = float(np.median(times))
+    return {"value": units_per_run / med, "unit": unit, "cores": cores, "kind": kind,
+            "sample": f"{sample}; median of {len(times)} runs ({med * 1e3:.3f} ms each) after 2 warm-ups",
+            **host_info()}
+
+
 def cpu_baseline(img, ins, angles, obs, budget_s=10.0):
     """The reference's CPU path (oracle/_ref, or the oracle port when the
     compiled reference is absent) on this host's cores: whole particle-filter
@@ -70,28 +112,111 @@ def cpu_baseline(img, ins, angles, obs, budget_s=10.0):
         def step():
             r = m.numpy_calc_range_angles(ins, angles)
             return m.eval_sensor_model(obs, r, BEAMS, PARTICLES)
-    step()  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        step()
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    rays = n * PARTICLES * BEAMS
-    out = {"value": rays / el, "unit": "rays/s", "cores": threads, "kind": kind,
-           "sample": f"{n} full steps ({PARTICLES}x{BEAMS} rays + sensor model) in {el:.1f} s"}
+    rays = PARTICLES * BEAMS
+    times = timed_runs(step, budget_s)
+    out = rate_entry(rays, times, "rays/s", threads, kind,
+                     f"full steps ({PARTICLES}x{BEAMS} rays + sensor model) on {threads} threads")
     if kind == "reference" and threads > 1:  # the reference's own single-threaded path as well
-        n1, t0 = 0, time.perf_counter()
-        while True:
-            m.pf_step(ins, angles, obs, 1)
-            n1 += 1
-            el1 = time.perf_counter() - t0
-            if el1 >= budget_s / 2:
-                break
-        out["single_thread"] = {"value": n1 * PARTICLES * BEAMS / el1, "unit": "rays/s", "cores": 1,
-                                "sample": f"{n1} full steps in {el1:.1f} s"}
+        t1 = timed_runs(lambda: m.pf_step(ins, angles, obs, 1), budget_s / 2)
+        out["single_thread"] = rate_entry(rays, t1, "rays/s", 1, kind, "full steps on 1 thread")
     return out
+
+
+def _stream_ctx():
+    import torch
+
+    import range_libc_amd as rl
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    ctx = rl.default_context(0)
+    ctx.set_stream(stream.cuda_stream)
+    return torch, rl, dev, stream, ctx
+
+
+def _kernel_ms(torch, stream, fn, reps):
+    """Mean time per launch of `reps` back-to-back launches of fn on `stream`,
+    bracketed by one HIP-event pair behind a spin kernel (no host gap enters)."""
+    fn()
+    torch.cuda._sleep(50_000_000)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_c1(args):
+    """The reference's CPU-runnable configuration: plain ray marching through
+    numpy_calc_range (RangeLib.h:439-480, 927-962) on a seeded 512x512 map of 25
+    rectangles and 15 disks with a 2-px border, 100,000 world-frame queries from
+    uniformly drawn free cells (sub-cell offsets, headings uniform in [0, 2 pi)),
+    max_range 500.  One step = one launch over the 100,000 device-resident
+    queries; every range is compared with the oracle."""
+    from oracle import oracle as O
+    from range_libc_amd import workloads as W
+
+    torch, rl, dev, stream, ctx = _stream_ctx()
+    N, R = 100_000, 500.0
+    img = W.rect_disk_map(512, seed=0)
+    q = W.world_poses(img, N, seed=5)
+    occ = O.xmajor(img)
+    rl.DeviceMap(ctx, occ[:64, :64].copy())  # EDT kernels' code loaded before the timed build
+    m = rl.DeviceMap(ctx, occ)
+    edt_ms = m.edt_build_ms()
+    d_q = torch.from_numpy(q).to(dev)
+    d_out = torch.empty(N, dtype=torch.float32, device=dev)
+    d_cnt = torch.zeros(N, dtype=torch.int32, device=dev)
+    m.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_WORLD, R, d_q.data_ptr(), 0, d_cnt.data_ptr(), N)
+    stream.synchronize()
+    reads = int(d_cnt.sum().item())
+    max_reads = int(d_cnt.max().item())
+    alg = 4 * reads + 16 * N  # 4 B per distance-map read, 12 B pose in, 4 B range out
+
+    def cast():
+        m.numpy_calc_range_dev(rl.RL_METHOD_RM, R, d_q.data_ptr(), d_out.data_ptr(), N)
+
+    for _ in range(args.warmup):
+        cast()
+    kern_ms = _kernel_ms(torch, stream, cast, args.steps)
+    got = d_out.cpu().numpy()
+    want = O.Oracle(img, R, "rm").numpy_calc_range(q)
+    mismatches = int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+    h_out = np.empty(N, np.float32)
+    host_t = timed_runs(lambda: m.numpy_calc_range(rl.RL_METHOD_RM, R, q, h_out, N), 1.0, min_runs=10)
+    assert np.array_equal(h_out.view(np.uint32), got.view(np.uint32))
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    result = {
+        "metric": "rays/s on 1 GPU (numpy_calc_range, plain ray marching, 512^2 rectangles+disks, 100k queries)",
+        "value": N / (kern_ms * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": kern_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c1_rect_disk512_100k_r500", "map": "512x512 rectangles+disks (seed 0)",
+                   "queries": N, "max_range": R, "parallelism": "single_gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_cast<RM, world>",
+                     "kernel_ms": kern_ms, "alg_bytes_per_launch": alg, "edt_reads_per_ray": reads / N,
+                     "max_reads_per_ray": max_reads},
+        "mismatches_vs_oracle": mismatches, "rays_checked": N,
+        "host_api_ms_per_call": float(np.median(host_t)) * 1e3,
+        "host_api_rays_per_s": N / float(np.median(host_t)),
+        "edt_build_ms": edt_ms,
+    }
+    if not args.no_cpu and O.ref_available():
+        threads = len(os.sched_getaffinity(0))
+        ref = O.Reference(img, R, "rm")
+        tm = timed_runs(lambda: ref.numpy_calc_range(q, threads), args.cpu_budget / 2)
+        t1 = timed_runs(lambda: ref.numpy_calc_range(q, 1), args.cpu_budget / 2)
+        result["cpu_baseline"] = rate_entry(N, tm, "rays/s", threads, "reference",
+                                            f"numpy_calc_range over all {N} queries on {threads} threads")
+        result["cpu_baseline"]["single_thread"] = rate_entry(N, t1, "rays/s", 1, "reference",
+                                                             f"numpy_calc_range over all {N} queries on 1 thread")
+        te = timed_runs(lambda: O.ref_edt(img), 2.0, min_runs=5, warmup=1)
+        result["cpu_baseline"]["edt_build_ms"] = float(np.median(te)) * 1e3
+    print(json.dumps(result))
 
 
 def bench_uneven64(args):
@@ -159,12 +284,74 @@ def bench_uneven64(args):
 
     per_map()
     per_map_ms = timed(per_map, max(1, args.steps // 4))
-    print(json.dumps({
+    # algorithmic bytes of one batch launch: per job 4 B per distance-map read
+    # (counted on the device), 12 B per pose, 8 B per beam (angle, observation),
+    # 4 B range out and 8 B table entry per ray, 8 B weight per particle
+    reads = 0
+    max_reads = 0
+    for j, (dm, ins, ang, obs, rng_, w) in zip(jobs, keep):
+        n, mm = j["num_particles"], j["num_angles"]
+        cnt = torch.zeros(n * mm, dtype=torch.int32, device=dev)
+        dm.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, MAX_RANGE, ins.data_ptr(), ang.data_ptr(),
+                           cnt.data_ptr(), n * mm, mm)
+        stream.synchronize()
+        reads += int(cnt.sum().item())
+        max_reads = max(max_reads, int(cnt.max().item()))
+    alg = 4 * reads + sum(12 * j["num_particles"] + 8 * j["num_angles"] + 12 * j["num_particles"] * j["num_angles"]
+                          + 8 * j["num_particles"] for j in jobs)
+    achieved = alg / (batch_ms * 1e-3) / 1e9
+    result = {
         "metric": "rays/s on 1 GPU (uneven 64-map batch, one launch)", "value": total_rays / (batch_ms * 1e-3),
         "unit": "rays/s", "ms_per_step": batch_ms, "per_map_launches_ms": per_map_ms,
         "speedup_vs_per_map": per_map_ms / batch_ms, "autotuned": batch.config(), "autotune_best_ms": tuned_ms,
         "rays": total_rays, "maps": len(jobs), "setup_s": setup_s, "steps": args.steps,
-        "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}))
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch<RM>",
+                     "kernel_ms": batch_ms, "alg_bytes_per_launch": alg, "edt_reads_per_ray": reads / total_rays,
+                     "max_reads_per_ray": max_reads},
+        "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}
+    if not args.no_cpu:
+        result["cpu_baseline"] = uneven64_cpu_baseline(specs, table, keep, args.cpu_budget)
+    print(json.dumps(result))
+
+
+def uneven64_cpu_baseline(specs, table, keep, budget_s):
+    """The reference's per-map loop (numpy_calc_range_angles + eval_sensor_model,
+    RangeLib.h:482-555) over the 64 maps of the batch, on the compiled
+    reference: every map's steps split over all host threads (median of 3 runs
+    per map), and on one thread for every eighth map; the distance transforms
+    are built beforehand and timed apart."""
+    from oracle import oracle as O
+    from range_libc_amd import workloads as W
+
+    if not O.ref_available():
+        return None
+    threads = len(os.sched_getaffinity(0))
+    t_mt = 0.0
+    t_1, rays_1 = 0.0, 0
+    build_s = 0.0
+    total = 0
+    for i, (sp, k) in enumerate(zip(specs, keep)):
+        img = W.make_map(sp["kind"], sp["side"], sp["map_seed"])
+        ins, ang, obs = (t.cpu().numpy() for t in k[1:4])
+        t0 = time.perf_counter()
+        ref = O.Reference(img, MAX_RANGE, "rm")
+        ref.set_sensor_model(table)
+        build_s += time.perf_counter() - t0
+        rays = ins.shape[0] * ang.shape[0]
+        t_mt += float(np.median(timed_runs(lambda: ref.pf_step(ins, ang, obs, threads), 0.0, min_runs=3,
+                                           warmup=1)))
+        if i % 8 == 0:
+            t_1 += float(np.median(timed_runs(lambda: ref.pf_step(ins, ang, obs, 1), 0.0, min_runs=1, warmup=0)))
+            rays_1 += rays
+        total += rays
+        del ref
+    return {"value": total / t_mt, "unit": "rays/s", "cores": threads, "kind": "reference",
+            "sample": f"all 64 maps, {total} rays, per-map steps on {threads} threads (median of 3 runs per map): "
+                      f"{t_mt:.3f} s per batch; distance transforms built beforehand in {build_s:.1f} s",
+            "single_thread": {"value": rays_1 / t_1, "unit": "rays/s", "cores": 1, "kind": "reference",
+                              "sample": f"every 8th map ({rays_1} rays) on 1 thread: {t_1:.3f} s"},
+            "edt_build_s_all_maps": build_s, **host_info()}
 
 
 def bench_surrogate(args):
@@ -486,7 +673,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
-    ap.add_argument("--workload", default="pf_step", choices=["pf_step", "uneven64", "surrogate", "glt", "cddt", "bresenham"],
+    ap.add_argument("--workload", default="pf_step",
+                    choices=["pf_step", "c1", "uneven64", "surrogate", "glt", "cddt", "bresenham"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
                          "surrogate (learned start point on the step's map)")
     ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
@@ -502,6 +690,8 @@ def main():
     import range_libc_amd as rl
 
     dev = torch.device("cuda", 0)
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "uneven64":
         return bench_uneven64(args)
     if args.workload == "surrogate":

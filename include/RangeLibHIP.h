@@ -3,15 +3,19 @@
 //
 // Same namespace, class names, constructors and method signatures as
 // RangeLib.h for the classes on the ray-casting path:
-//   ranges::OMap            (RangeLib.h:121-322, grid and world fields)
+//   ranges::OMap            (RangeLib.h:121-322: grid, world fields, PNG
+//                            constructors and save)
 //   ranges::RayMarchingGPU  (RangeLib.h:774-920)
 //   ranges::RayMarching     (RangeLib.h:922-969)
 //   ranges::BresenhamsLine  (RangeLib.h:691-772)
-// Every method runs on the GPU.  Differences from the reference:
+//   ranges::CDDTCast        (RangeLib.h:971-1770: tables built and pruned on the GPU)
+//   ranges::GiantLUTCast    (RangeLib.h:1772-1900: table built on the GPU)
+// Every casting method runs on the GPU.  Differences from the reference:
 //   * failures throw std::runtime_error carrying rl_last_error() (the
 //     reference prints, or throws std::string when built without CUDA);
-//   * OMap has no PNG constructor (map ingestion is not on this path: fill
-//     `grid` or use OMap(w, h));
+//   * OMap(filename) keeps no raw_grid of gray values (nothing on the
+//     casting path reads it), and a PNG that fails to decode leaves a 1 x 1
+//     free grid with has_error set (the reference leaves the grid empty);
 //   * the GPU sensor-model call computes the weights (the reference's GPU
 //     path prints "unimplemented", kernels.cu:281-315).
 // Link with librangelib_hip.so.
@@ -19,6 +23,7 @@
 #define RANGE_LIB_HIP_H
 
 #include <stdint.h>
+#include <stdio.h>
 
 #include <iostream>
 #include <memory>
@@ -63,6 +68,40 @@ struct OMap {
       : has_error(false), width(w), height(h), grid(w, std::vector<bool>(h, false)), fn(""),
         world_scale(1.0f), world_angle(0.0f), world_origin_x(0.0f), world_origin_y(0.0f),
         world_sin_angle(0.0f), world_cos_angle(1.0f) {}
+
+  // RangeLib.h:158-201: occupied where the pixel's gray value is below the
+  // threshold (the PNG is decoded by the library, rl_png_occupancy)
+  OMap(std::string filename) : OMap(filename, 128) {}
+  OMap(std::string filename, float threshold)
+      : has_error(false), width(0), height(0), fn(filename), world_scale(1.0f), world_angle(0.0f),
+        world_origin_x(0.0f), world_origin_y(0.0f), world_sin_angle(0.0f), world_cos_angle(1.0f) {
+    int w = 0, h = 0;
+    std::vector<uint8_t> occ;
+    int rc = rl_png_occupancy(filename.c_str(), threshold, nullptr, &w, &h);
+    if (rc == RL_OK) {
+      occ.resize((size_t)w * h);
+      rc = rl_png_occupancy(filename.c_str(), threshold, occ.data(), &w, &h);
+    }
+    if (rc != RL_OK) {
+      std::printf("ERROR %d: %s\n", rc, rl_last_error());
+      has_error = true;
+      w = h = 1;
+      occ.assign(1, 0);
+    }
+    width = (unsigned)w;
+    height = (unsigned)h;
+    grid.assign(width, std::vector<bool>(height, false));
+    for (unsigned x = 0; x < width; ++x)
+      for (unsigned y = 0; y < height; ++y) grid[x][y] = occ[(size_t)x * height + y] != 0;
+  }
+
+  // RangeLib.h:264-291: opaque RGBA PNG, black where occupied; returns true on error
+  bool save(std::string filename) {
+    std::vector<uint8_t> occ = xmajor();
+    const int rc = rl_png_save_occupancy(filename.c_str(), occ.data(), (int)width, (int)height);
+    if (rc != RL_OK) std::cout << "encoder error " << rc << ": " << rl_last_error() << std::endl;
+    return rc != RL_OK;
+  }
 
   bool get(int x, int y) { return grid[x][y]; }
   bool isOccupied(int x, int y) {
@@ -161,6 +200,32 @@ class BresenhamsLine : public GpuRangeMethod {
   BresenhamsLine(OMap m, float mr) : GpuRangeMethod(m, mr, RL_METHOD_BL) {}
   float calc_range(float x, float y, float heading) { return single(x, y, heading); }  // RangeLib.h:696
   int memory() { return 0; }
+};
+
+class CDDTCast : public GpuRangeMethod {
+ public:
+  // RangeLib.h:974-1174: the compressed tables are built on the GPU
+  CDDTCast(OMap m, float mr, unsigned int td) : GpuRangeMethod(m, mr, RL_METHOD_CDDT) {
+    detail::check(rl_map_build_cddt(map_, (int)td));
+  }
+  float calc_range(float x, float y, float heading) { return single(x, y, heading); }  // RangeLib.h:1342
+  // RangeLib.h:1176-1290
+  void prune(float max_range) { detail::check(rl_map_prune_cddt(map_, max_range)); }
+  // RangeLib.h:616-676
+  void calc_range_many_radial_optimized(float* ins, float* outs, int num_particles, int num_rays, float min_angle,
+                                        float max_angle) {
+    detail::check(rl_calc_range_many_radial_optimized(map_, method_, max_range_, ins, outs, num_particles, num_rays,
+                                                      min_angle, max_angle));
+  }
+};
+
+class GiantLUTCast : public GpuRangeMethod {
+ public:
+  // RangeLib.h:1781-1823: the table is built on the GPU
+  GiantLUTCast(OMap m, float mr, unsigned int td) : GpuRangeMethod(m, mr, RL_METHOD_GLT) {
+    detail::check(rl_map_build_giant_lut(map_, mr, (int)td));
+  }
+  float calc_range(float x, float y, float heading) { return single(x, y, heading); }  // RangeLib.h:1869
 };
 
 }  // namespace ranges

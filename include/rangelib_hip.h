@@ -61,6 +61,27 @@ typedef struct rl_map rl_map;
 /* Message of the last failed call on this thread ("" if none). */
 const char* rl_last_error(void);
 int rl_abi_version(void);
+/* Loads the sinf/cosf exception table (no GPU needed) and writes the C library
+ * release it was generated with and the one this process runs (len bytes
+ * each; len 0 writes nothing).  RL_ERR_STATE when they differ: the reference's
+ * sinf/cosf (RangeLib.h:713-714,931-932) come from the running C library, so
+ * bit parity needs the same release; rl_create refuses the same mismatch unless
+ * RL_TRIG_ALLOW_LIBC_MISMATCH=1 is set.  No counterpart in the reference. */
+int rl_trig_table_info(char* table_libc, char* running_libc, int len);
+
+/* --- map ingestion (host only; no GPU needed) ------------------------------ */
+/* OMap(filename, threshold) (RangeLib.h:158-201): decodes the PNG at `path` to
+ * 8-bit RGBA and writes the occupancy x-major, occ[x*height + y] = 1 where the
+ * pixel's gray value (int)(0.229 r + 0.587 g + 0.114 b) (RangeUtils.h:30-32, with
+ * r, g, b the third, second and first byte of the pixel as the reference reads
+ * them) is below `threshold` (128 in OMap(filename)).  With occ_xmajor NULL only
+ * *width / *height are written (size query).  RL_ERR_IO when the file cannot be
+ * read or decoded (the reference prints the decoder error and flags has_error). */
+int rl_png_occupancy(const char* path, float threshold, uint8_t* occ_xmajor, int* width, int* height);
+/* OMap::save (RangeLib.h:264-291): an opaque RGBA PNG, black where occupied,
+ * white elsewhere.  RL_OK on success (the reference returns its encoder's
+ * error code, 0 on success). */
+int rl_png_save_occupancy(const char* path, const uint8_t* occ_xmajor, int width, int height);
 
 /* --- context ------------------------------------------------------------ */
 /* Binds a HIP device, creates a stream, uploads the sinf/cosf exception table.

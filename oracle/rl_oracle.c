@@ -317,10 +317,14 @@ void or_numpy_calc_range_angles(const or_method* m, const float* ins, const floa
   }
 }
 
-/* std::min<float>(std::max<float>(v, 0.0), (float)K - 1.0) */
+/* std::min<float>(std::max<float>(v, 0.0), (float)K - 1.0) (RangeLib.h:547-550).
+ * A NaN passes through the reference's min/max and its (int) conversion then
+ * indexes outside the table (undefined behaviour); here NaN gives K - 1, the
+ * max-range row/column, as a no-return beam reads. */
 static float clamp_table(float v, int K) {
-  float lo = (v < 0.0f) ? 0.0f : v;
   float hi = (float)((double)(float)K - 1.0);
+  if (v != v) return hi;
+  float lo = (v < 0.0f) ? 0.0f : v;
   return (hi < lo) ? hi : lo;
 }
 

@@ -41,16 +41,39 @@ LRU_CACHE_SIZE = 1000000
 SHOULD_USE_CUDA = True  # the GPU class is always built here
 
 
-def _rgb2gray_png(path: str) -> np.ndarray:
-    """Grayscale of a PNG as OMap(filename) computes it (RangeLib.h:186-199):
-    channels read as (r, g, b) = (B, G, R) of the decoded RGBA image, then
-    (int)(float)(0.229 r + 0.587 g + 0.114 b) (RangeUtils.h:30-32)."""
-    from PIL import Image  # optional dependency of the PNG constructor only
+def _png_occupancy(path: str, threshold: float) -> np.ndarray:
+    """OMap(filename, threshold) (RangeLib.h:158-201) through the library's PNG
+    decoder: x-major uint8 occupancy, 1 where the pixel's gray value is below
+    the threshold."""
+    import ctypes as C
 
-    rgba = np.asarray(Image.open(path).convert("RGBA"), dtype=np.float64)
-    r, g, b = rgba[..., 2], rgba[..., 1], rgba[..., 0]
-    gray = (0.229 * r + 0.587 * g + 0.114 * b).astype(np.float32)
-    return gray.astype(np.int64)
+    h = _lib.lib()
+    w, hh = C.c_int(), C.c_int()
+    _lib.check(h.rl_png_occupancy(path.encode(), threshold, None, C.byref(w), C.byref(hh)))
+    occ = np.empty((w.value, hh.value), dtype=np.uint8)
+    _lib.check(h.rl_png_occupancy(path.encode(), threshold, occ.ctypes.data, C.byref(w), C.byref(hh)))
+    return occ
+
+
+def ros_yaw(q) -> float:
+    """Yaw of a quaternion message (x, y, z, w) as the reference computes it
+    (RangeLibc.pyx:120-128): tf.transformations.euler_from_quaternion with the
+    default static 'sxyz' axes, i.e. the quaternion's rotation matrix
+    (normalised, scaled by sqrt(2 / |q|^2)) followed by
+    atan2(M[1, 0], M[0, 0]), or 0 at the gimbal lock (|cos pitch| <= 4 eps)."""
+    qv = np.array([float(q.x), float(q.y), float(q.z), float(q.w)], dtype=np.float64)
+    eps4 = np.finfo(float).eps * 4.0
+    nq = float(np.dot(qv, qv))
+    if nq < eps4:
+        return 0.0  # identity rotation
+    qv *= math.sqrt(2.0 / nq)
+    o = np.outer(qv, qv)
+    m00 = 1.0 - o[1, 1] - o[2, 2]
+    m10 = o[0, 1] + o[2, 3]
+    cy = math.sqrt(m00 * m00 + m10 * m10)
+    if cy > eps4:
+        return math.atan2(m10, m00)
+    return 0.0  # gimbal lock: the 'sxyz' decomposition puts the rotation in roll
 
 
 class PyOMap:
@@ -92,13 +115,11 @@ class PyOMap:
     def _load_png(self, path: str, threshold: float):
         self.fn = path
         try:
-            gray = _rgb2gray_png(path)
-        except Exception as e:  # OMap prints the decoder error and flags has_error
+            self.occ = _png_occupancy(path, threshold)
+        except RangeLibError as e:  # OMap prints the decoder error and flags has_error
             print(f"ERROR: {e}")
             self._error = True
             self.occ = np.zeros((1, 1), dtype=np.uint8)
-            return
-        self.occ = np.ascontiguousarray((gray < threshold).T.astype(np.uint8))
 
     def _load_ros(self, msg):
         """nav_msgs/OccupancyGrid (RangeLibc.pyx:146-167): cells > 10 occupied,
@@ -107,9 +128,7 @@ class PyOMap:
         width, height = int(info.width), int(info.height)
         arr = np.asarray(msg.data).reshape((height, width))
         self.occ = np.ascontiguousarray((arr > 10).astype(np.uint8))  # occ[x=row, y=col]
-        q = info.origin.orientation
-        yaw = math.atan2(2.0 * (q.w * q.z + q.x * q.y), 1.0 - 2.0 * (q.y * q.y + q.z * q.z))
-        angle = -1.0 * yaw
+        angle = -1.0 * ros_yaw(info.origin.orientation)
         self.world = (float(info.resolution), angle, float(info.origin.position.x),
                       float(info.origin.position.y), float(np.sin(angle)), float(np.cos(angle)))
 
@@ -122,12 +141,11 @@ class PyOMap:
     def save(self, fn) -> bool:
         """RGBA PNG, opaque, black where occupied and white elsewhere (OMap::save,
         RangeLib.h:264-291); returns False on success as the reference's error code."""
-        from PIL import Image
-
-        v = np.where(self.occ.T != 0, 0, 255).astype(np.uint8)
-        rgba = np.stack([v, v, v, np.full_like(v, 255)], axis=-1)
-        Image.fromarray(rgba, mode="RGBA").save(fn)
-        return False
+        occ = np.ascontiguousarray(self.occ, dtype=np.uint8)
+        rc = _lib.lib().rl_png_save_occupancy(str(fn).encode(), occ.ctypes.data, occ.shape[0], occ.shape[1])
+        if rc != 0:
+            print(f"encoder error: {_lib.lib().rl_last_error().decode()}")
+        return rc != 0
 
     def isOccupied(self, x: int, y: int) -> bool:
         return bool(self.occ[x, y])

@@ -34,6 +34,9 @@ _u8p = C.POINTER(C.c_uint8)
 # name -> argument types; every function returns int status
 SIGNATURES = {
     "rl_abi_version": [],
+    "rl_trig_table_info": [C.c_char_p, C.c_char_p, C.c_int],
+    "rl_png_occupancy": [C.c_char_p, C.c_float, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "rl_png_save_occupancy": [C.c_char_p, _vp, C.c_int, C.c_int],
     "rl_create": [C.c_int, C.POINTER(_vp)],
     "rl_destroy": [_vp],
     "rl_set_stream": [_vp, _vp],

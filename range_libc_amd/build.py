@@ -6,6 +6,7 @@
 Outputs (git-ignored, shipped with the source tree):
   range_libc_amd/librangelib_hip.so        the C-ABI library (gfx950)
   range_libc_amd/_data/trig_exc.bin        sinf/cosf exception table
+  range_libc_amd/pywrapper/range_libc.*.so the Cython module over the C ABI
 The trig table depends only on rl_trig.h, its generator and the host C
 library, so it is regenerated only when one of those changes.
 """
@@ -76,9 +77,30 @@ def build_trig_table(force: bool = False) -> str:
 def build_library() -> str:
     src = os.path.join(CSRC, "rl_lib.hip")
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, src, "-o", tmp, "-lz"])
     os.replace(tmp, LIB)
     return LIB
+
+
+def build_cython() -> str:
+    """The Cython module `range_libc` (pywrapper/RangeLibc.pyx) over the C ABI,
+    linked against the library in this directory."""
+    import sysconfig
+
+    import numpy as np
+
+    pyx = os.path.join(PKG, "pywrapper", "RangeLibc.pyx")
+    build_dir = os.path.join(PKG, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    c_file = os.path.join(build_dir, "range_libc.c")
+    out = os.path.join(PKG, "pywrapper", "range_libc" + sysconfig.get_config_var("EXT_SUFFIX"))
+    _run([sys.executable, "-m", "cython", "-3", "--module-name", "range_libc", "-o", c_file, pyx])
+    tmp = out + ".tmp"
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-w", "-DNPY_NO_DEPRECATED_API=NPY_1_7_API_VERSION",
+          "-I", sysconfig.get_paths()["include"], "-I", np.get_include(), "-I", os.path.join(ROOT, "include"),
+          c_file, "-o", tmp, "-L", PKG, "-lrangelib_hip", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, out)
+    return out
 
 
 def build(force_table: bool = False):
@@ -86,6 +108,7 @@ def build(force_table: bool = False):
         raise RuntimeError(f"hipcc not found ({HIPCC})")
     build_trig_table(force_table)
     build_library()
+    build_cython()
 
 
 if __name__ == "__main__":

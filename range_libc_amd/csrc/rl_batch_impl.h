@@ -72,8 +72,9 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
 struct rl_batch {
   rl_ctx* ctx = nullptr;
   int method = RL_METHOD_RM;
-  std::vector<rl_batch_job> jobs;
+  std::vector<rl_batch_job> jobs;     // each job's map holds one reference per job
   std::vector<double> reads_per_ray;  // measured per job
+  std::vector<unsigned long long> gens;  // each job's map generation at the last upload
   // launch configuration
   int rays_per_wave = 64;  // upper bound on S
   int lds_box = 0;
@@ -93,6 +94,8 @@ int batch_upload(rl_batch* b) {
   const int nj = (int)b->jobs.size();
   std::vector<BatchMap> maps(nj);
   std::vector<std::pair<double, BatchItem>> items;
+  b->gens.resize(nj);
+  for (int i = 0; i < nj; ++i) b->gens[i] = b->jobs[i].map->gen.load();
   for (int i = 0; i < nj; ++i) {
     const rl_batch_job& j = b->jobs[i];
     BatchMap& bm = maps[i];
@@ -155,7 +158,29 @@ int batch_upload(rl_batch* b) {
   return RL_OK;
 }
 
+// Releases the map references of the current job list.
+void batch_release_jobs(rl_batch* b) {
+  for (const rl_batch_job& j : b->jobs) rl_map_destroy(j.map);
+  b->jobs.clear();
+  b->gens.clear();
+}
+
+// A job's map changed since the descriptors were uploaded (a setter replaced a
+// buffer or the world frame): upload them again, which a graph capture cannot
+// do.
+int batch_refresh(rl_batch* b) {
+  bool stale = false;
+  for (size_t i = 0; i < b->jobs.size(); ++i) stale = stale || b->jobs[i].map->gen.load() != b->gens[i];
+  if (!stale) return RL_OK;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(b->ctx->stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+    return fail(RL_ERR_STATE, "a map of this batch changed since its descriptors were uploaded; "
+                              "launch once outside the capture first");
+  return batch_upload(b);
+}
+
 int batch_launch_locked(rl_batch* b) {
+  TRY(batch_refresh(b));
   if (b->n_items == 0) return RL_OK;
   rl_ctx* c = b->ctx;
   int grid = b->grid > 0 && b->grid < b->n_items ? b->grid : b->n_items;
@@ -217,6 +242,7 @@ int rl_batch_destroy(rl_batch* b) {
   if (b->d_maps) (void)hipFree(b->d_maps);
   if (b->d_items) (void)hipFree(b->d_items);
   rl_ctx* c = b->ctx;
+  batch_release_jobs(b);
   delete b;
   return rl_destroy(c);  // drops the batch's reference
 }
@@ -239,6 +265,10 @@ int rl_batch_set_jobs(rl_batch* b, const rl_batch_job* jobs, int n_jobs) {
   }
   std::lock_guard<std::mutex> lk(b->ctx->mu);
   TRY(use_device(b->ctx));
+  // the batch keeps its maps alive: a map destroyed by its caller is freed
+  // when the batch lets it go (new jobs or rl_batch_destroy)
+  for (int i = 0; i < n_jobs; ++i) jobs[i].map->refs.fetch_add(1);
+  batch_release_jobs(b);
   b->jobs.assign(jobs, jobs + n_jobs);
   b->reads_per_ray.assign(n_jobs, 1.0);
   for (int i = 0; i < n_jobs; ++i) TRY(batch_probe_reads(b, i, &b->reads_per_ray[i]));

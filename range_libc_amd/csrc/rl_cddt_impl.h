@@ -472,6 +472,7 @@ int rl_map_build_cddt(rl_map* m, int theta_disc) {
   std::lock_guard<std::mutex> lk(c->mu);
   TRY(use_device(c));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  m->gen.fetch_add(1);
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -497,6 +498,7 @@ int rl_map_prune_cddt(rl_map* m, float max_range) {
   TRY(use_device(c));
   if (!m->cddt_vals) return fail(RL_ERR_STATE, "no CDDT tables (call rl_map_build_cddt first)");
   HIP_TRY(hipStreamSynchronize(c->stream));
+  m->gen.fetch_add(1);
   return cddt_prune_locked(m, max_range);
 }
 

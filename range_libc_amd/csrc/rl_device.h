@@ -705,10 +705,14 @@ __device__ __forceinline__ void to_grid(const WorldDev& w, float xw, float yw, f
   *gy = y;
 }
 
-// std::min<float>(std::max<float>(v, 0.0), (float)K - 1.0)  (RangeLib.h:547-550)
+// std::min<float>(std::max<float>(v, 0.0), (float)K - 1.0)  (RangeLib.h:547-550).
+// A NaN passes through the reference's min/max and its (int) conversion then
+// indexes outside the table (undefined behaviour); here NaN gives K - 1, the
+// max-range row/column, so the table read stays in bounds.
 __device__ __forceinline__ float clamp_table(float v, float hi) {
   float lo = (v < 0.0f) ? 0.0f : v;
-  return (hi < lo) ? hi : lo;
+  lo = (hi < lo) ? hi : lo;
+  return (v != v) ? hi : lo;
 }
 
 }  // namespace rl

@@ -15,6 +15,7 @@
 //
 // Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off (see build.py).
 #include <dlfcn.h>
+#include <gnu/libc-version.h>
 #include <hip/hip_runtime.h>
 #include <libgen.h>
 #include <stdarg.h>
@@ -29,6 +30,7 @@
 
 #include "../../include/rangelib_hip.h"
 #include "rl_device.h"
+#include "rl_png.h"
 
 #pragma clang fp contract(off)
 
@@ -788,6 +790,23 @@ const TrigFile* trig_table(std::string* err) {
   return &g_trig;
 }
 
+// The exception table lists where the C library that generated it rounds
+// sinf/cosf away from the nearest float; the reference calls the C library of
+// the running process, so the two must be the same release for bit parity.
+// RL_TRIG_ALLOW_LIBC_MISMATCH=1 accepts another release (results may then
+// differ from the reference's in the last place of a ray direction).
+int check_trig_libc(const TrigFile* tf) {
+  const char* running = gnu_get_libc_version();
+  if (tf->libc_version == running) return RL_OK;
+  const char* allow = getenv("RL_TRIG_ALLOW_LIBC_MISMATCH");
+  if (allow && strcmp(allow, "1") == 0) return RL_OK;
+  return fail(RL_ERR_STATE,
+              "sinf/cosf exception table was generated with C library %s but this process runs %s: "
+              "rebuild the table (python -m range_libc_amd.build --force) or set "
+              "RL_TRIG_ALLOW_LIBC_MISMATCH=1",
+              tf->libc_version.c_str(), running);
+}
+
 int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
 
 int segment_size(int m) {
@@ -835,7 +854,12 @@ struct rl_ctx {
 
 struct rl_map {
   rl_ctx* ctx = nullptr;
-  std::atomic<int> refs{1};  // the caller's handle plus one per surrogate of this map
+  // the caller's handle plus one per surrogate and per batch job list holding this map
+  std::atomic<int> refs{1};
+  // bumped whenever a device buffer or setting that a launch descriptor copies
+  // changes (world frame, sensor table, GiantLUTCast and CDDT tables): batches
+  // re-upload their descriptors when it moved
+  std::atomic<unsigned long long> gen{0};
   int width = 0, height = 0;
   float* d_edt = nullptr;
   uint8_t* d_occ = nullptr;
@@ -1158,6 +1182,61 @@ const char* rl_last_error(void) { return g_err.c_str(); }
 
 int rl_abi_version(void) { return RL_ABI_VERSION; }
 
+int rl_png_occupancy(const char* path, float threshold, uint8_t* occ_xmajor, int* width, int* height) {
+  CHECK_ARG(path && width && height, "null argument");
+  rlpng::Image img;
+  std::string err;
+  if (!rlpng::decode_file(path, &img, &err)) return fail(RL_ERR_IO, "%s: %s", path, err.c_str());
+  if (img.width > 65535 || img.height > 65535)
+    return fail(RL_ERR_ARG, "%s: map size %ux%u outside 1..65535", path, img.width, img.height);
+  *width = (int)img.width;
+  *height = (int)img.height;
+  if (!occ_xmajor) return RL_OK;
+  const size_t W = img.width, H = img.height;
+  for (size_t y = 0; y < H; ++y)
+    for (size_t x = 0; x < W; ++x) {
+      const uint8_t* px = &img.rgba[4 * (y * W + x)];
+      // channels taken as (r, g, b) = (third, second, first) byte, as
+      // OMap(filename, threshold) reads them (RangeLib.h:190-193); gray =
+      // 0.229 r + 0.587 g + 0.114 b in double, returned as float and
+      // truncated (RangeUtils.h:30-32, RangeLib.h:194)
+      double v = 0.229 * (double)px[2];
+      v = v + 0.587 * (double)px[1];
+      v = v + 0.114 * (double)px[0];
+      const int gray = (int)(float)v;
+      occ_xmajor[x * H + y] = ((float)gray < threshold) ? 1 : 0;
+    }
+  return RL_OK;
+}
+
+int rl_png_save_occupancy(const char* path, const uint8_t* occ_xmajor, int width, int height) {
+  CHECK_ARG(path && occ_xmajor, "null argument");
+  CHECK_ARG(width >= 1 && height >= 1, "bad map size");
+  const size_t W = (size_t)width, H = (size_t)height;
+  std::vector<uint8_t> rgba(4 * W * H, 255);  // opaque white (RangeLib.h:274-277)
+  for (size_t y = 0; y < H; ++y)
+    for (size_t x = 0; x < W; ++x)
+      if (occ_xmajor[x * H + y]) {  // black where occupied (RangeLib.h:279-283)
+        uint8_t* px = &rgba[4 * (y * W + x)];
+        px[0] = px[1] = px[2] = 0;
+      }
+  std::string err;
+  if (!rlpng::encode_file(path, rgba.data(), (unsigned)W, (unsigned)H, &err)) return fail(RL_ERR_IO, "%s", err.c_str());
+  return RL_OK;
+}
+
+int rl_trig_table_info(char* table_libc, char* running_libc, int len) {
+  CHECK_ARG(len >= 0 && (len == 0 || (table_libc && running_libc)), "bad argument");
+  std::string err;
+  const TrigFile* tf = trig_table(&err);
+  if (!tf) return fail(RL_ERR_IO, "%s", err.c_str());
+  if (len > 0) {
+    snprintf(table_libc, (size_t)len, "%s", tf->libc_version.c_str());
+    snprintf(running_libc, (size_t)len, "%s", gnu_get_libc_version());
+  }
+  return check_trig_libc(tf);
+}
+
 int rl_create(int device, rl_ctx** out) {
   CHECK_ARG(out, "out is NULL");
   *out = nullptr;
@@ -1167,6 +1246,7 @@ int rl_create(int device, rl_ctx** out) {
   std::string err;
   const TrigFile* tf = trig_table(&err);
   if (!tf) return fail(RL_ERR_IO, "%s", err.c_str());
+  TRY(check_trig_libc(tf));
   HIP_TRY(hipSetDevice(device));
   rl_ctx* c = new rl_ctx;
   c->device = device;
@@ -1299,6 +1379,7 @@ int rl_map_build_giant_lut(rl_map* m, float max_range, int theta_disc) {
   TRY(use_device(c));
   if (!m->d_edt) return fail(RL_ERR_STATE, "map has no distance transform");
   const long long total = (long long)m->width * m->height * theta_disc;
+  m->gen.fetch_add(1);
   if (m->d_glt) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipFree(m->d_glt));
@@ -1431,6 +1512,7 @@ int rl_map_set_world(rl_map* m, float scale, float angle, float ox, float oy, fl
   CHECK_ARG(m, "map is NULL");
   std::lock_guard<std::mutex> lk(m->ctx->mu);
   set_world(m, scale, angle, ox, oy, sin_a, cos_a);
+  m->gen.fetch_add(1);
   return RL_OK;
 }
 
@@ -1439,6 +1521,7 @@ int rl_map_set_sensor_table(rl_map* m, const double* table, int k) {
   CHECK_ARG(k >= 1, "table width must be >= 1");
   std::lock_guard<std::mutex> lk(m->ctx->mu);
   TRY(use_device(m->ctx));
+  m->gen.fetch_add(1);
   if (m->d_table) {
     HIP_TRY(hipStreamSynchronize(m->ctx->stream));
     HIP_TRY(hipFree(m->d_table));

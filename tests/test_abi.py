@@ -2,8 +2,11 @@
 
 No compute calls: loading librangelib_hip.so needs no GPU.
 """
+import ctypes as C
 import os
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -60,3 +63,43 @@ def test_module_flags_mirror_reference():
     assert rl.USE_CACHED_TRIG is False and rl.USE_ALTERNATE_MOD is True
     assert rl.USE_LRU_CACHE is False and rl.LRU_CACHE_SIZE == 1000000
     assert rl.SHOULD_USE_CUDA is True
+
+
+def test_trig_table_libc_version_matches():
+    h = _lib.lib()
+    a, b = C.create_string_buffer(64), C.create_string_buffer(64)
+    assert h.rl_trig_table_info(a, b, 64) == 0, h.rl_last_error()
+    assert a.value == b.value and a.value
+
+
+_LIBC_CHECK = r"""
+import ctypes as C, sys
+sys.path.insert(0, sys.argv[1])
+from range_libc_amd import _lib
+h = _lib.lib()
+a, b = C.create_string_buffer(64), C.create_string_buffer(64)
+rc = h.rl_trig_table_info(a, b, 64)
+print(rc, a.value.decode(), b.value.decode(), h.rl_last_error().decode(), sep="|")
+"""
+
+
+def test_trig_table_from_another_libc_is_refused(tmp_path):
+    """A table generated with another C library release is reported (and
+    rl_create refuses it) unless the mismatch is explicitly allowed."""
+    src = os.path.join(_lib.PKG, "_data", "trig_exc.bin")
+    dst = tmp_path / "trig_exc.bin"
+    data = bytearray(open(src, "rb").read())
+    # header: magic[8], 4 x uint32, 2 x double, libc_version[32]
+    off = 8 + 16 + 16
+    data[off:off + 32] = b"1.99".ljust(32, b"\0")
+    dst.write_bytes(bytes(data))
+    env = dict(os.environ, RL_TRIG_TABLE=str(dst))
+    env.pop("RL_TRIG_ALLOW_LIBC_MISMATCH", None)
+    out = subprocess.run([sys.executable, "-c", _LIBC_CHECK, ROOT], env=env, capture_output=True, text=True,
+                         check=True).stdout.strip().split("|")
+    assert out[0] == "3" and out[1] == "1.99" and out[2] != "1.99"
+    assert "1.99" in out[3] and "RL_TRIG_ALLOW_LIBC_MISMATCH" in out[3]
+    env["RL_TRIG_ALLOW_LIBC_MISMATCH"] = "1"
+    out = subprocess.run([sys.executable, "-c", _LIBC_CHECK, ROOT], env=env, capture_output=True, text=True,
+                         check=True).stdout.strip().split("|")
+    assert out[0] == "0"

@@ -235,7 +235,9 @@ def test_rm_special_headings_and_origins(ctx, rect_map, flags):
     outside the map, and max_range <= 0, all bit-exact with read counts."""
     flags = rl._lib.RL_MAP_NO_COMPACT if flags == "no_compact" else 0
     m = rl.DeviceMap(ctx, O.xmajor(rect_map), flags)
-    # |heading| < 1.6e6: the range where sinf/cosf follow the C library (see rl_trig.h)
+    # bit parity of the directions holds for |heading| <= 64 (sinf/cosf equal the C
+    # library's there); the large headings below were checked to round like the C
+    # library as well, so their rays stay comparable bit for bit
     heads = np.array([np.nan, np.inf, -np.inf, 1e6, -777777.0, 3e4, -3e4, 2 * np.pi, -0.0, 0.0, 1e-30, 4.0],
                      np.float32)
     origins = np.array([[100.5, 200.25], [-0.5, 30.0], [30.0, -0.75], [0.0, 0.0], [511.9, 511.9], [512.0, 3.0],
@@ -428,3 +430,28 @@ def test_scheduling_options(xcd, box, corridor_map):
                                       out.data_ptr(), 2500, 45)
         ctx.sync()
         assert_bits_equal(out.cpu().numpy(), want, f"xcd={xcd} box={box}")
+
+
+def test_sensor_model_nan_observations_and_ranges(corridor_map):
+    """NaN observations (no-return beams) and NaN expected ranges index the
+    table's max-range row / column on the GPU exactly as in the oracle, in
+    eval_sensor_model, the fused call and the particle-filter step."""
+    table = W.sensor_table(301)
+    n, m, R = 500, 60, 300.0
+    ins = W.world_poses(corridor_map, n, seed=31)
+    ang = W.beam_angles(m)
+    orc = O.Oracle(corridor_map, R, "rm")
+    orc.set_sensor_model(table)
+    obs = W.noisy_observation(orc.numpy_calc_range_angles(W.world_poses(corridor_map, 1, seed=32), ang), seed=33,
+                              max_range=R)
+    obs[[0, 7, 59]] = np.nan
+    ranges = orc.numpy_calc_range_angles(ins, ang)
+    ranges[::97] = np.nan
+    meth = rl.PyRayMarchingGPU(rl.PyOMap(corridor_map), R)
+    meth.set_sensor_model(table)
+    w = np.empty(n, np.float64)
+    meth.eval_sensor_model(obs, ranges, w, m, n)
+    assert np.array_equal(w, orc.eval_sensor_model(obs, ranges, m, n))
+    wf = np.empty(n, np.float64)
+    meth.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs, wf)
+    assert np.array_equal(wf, orc.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))

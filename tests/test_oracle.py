@@ -562,3 +562,20 @@ def test_cddt_json_golden(name):
         t = O.Oracle(img, R, "cddt", theta_disc=td, prune=prune).cddt_tables()
         text = rl.cddt_json(td, t["trans"], R, "", O.xmajor(img), t["widths"], t["start"], t["vals"])
         assert hashlib.sha256(text.encode()).hexdigest() == str(g[f"{name}_{tag}_json_sha"])
+
+
+def test_sensor_model_nan_index_is_max_row_and_column():
+    """A NaN observation or expected range selects the table's last row /
+    column (the reference indexes outside its table there)."""
+    img = np.zeros((16, 16), np.uint8)
+    K = 9
+    table = (np.arange(K * K, dtype=np.float64).reshape(K, K) + 1.0) / 64.0
+    orc = O.Oracle(img, 8.0, "rm")
+    orc.set_sensor_model(table)
+    obs = np.array([np.nan, 2.0, 3.5], np.float32)
+    ranges = np.array([1.0, np.nan, 7.9, np.nan, np.nan, 100.0], np.float32)
+    w = orc.eval_sensor_model(obs, ranges, 3, 2)
+    want0 = table[K - 1, 1] * table[2, K - 1] * table[3, 7]
+    want1 = table[K - 1, K - 1] * table[2, K - 1] * table[3, K - 1]
+    assert w[0] == ((1.0 * table[K - 1, 1]) * table[2, K - 1]) * table[3, 7] and np.isclose(w[0], want0)
+    assert w[1] == ((1.0 * table[K - 1, K - 1]) * table[2, K - 1]) * table[3, K - 1] and np.isclose(w[1], want1)
