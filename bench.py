@@ -292,8 +292,9 @@ def bench_uneven64(args):
     for j, (dm, ins, ang, obs, rng_, w) in zip(jobs, keep):
         n, mm = j["num_particles"], j["num_angles"]
         cnt = torch.zeros(n * mm, dtype=torch.int32, device=dev)
+        # num_particles is the pose count: the call writes n * mm counts
         dm.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, MAX_RANGE, ins.data_ptr(), ang.data_ptr(),
-                           cnt.data_ptr(), n * mm, mm)
+                           cnt.data_ptr(), n, mm)
         stream.synchronize()
         reads += int(cnt.sum().item())
         max_reads = max(max_reads, int(cnt.max().item()))

@@ -6,7 +6,7 @@
 Outputs (git-ignored, shipped with the source tree):
   range_libc_amd/librangelib_hip.so        the C-ABI library (gfx950)
   range_libc_amd/_data/trig_exc.bin        sinf/cosf exception table
-  range_libc_amd/pywrapper/range_libc.*.so the Cython module over the C ABI
+  range_libc_amd/pywrapper/_range_libc.*.so the Cython module over the C ABI
 The trig table depends only on rl_trig.h, its generator and the host C
 library, so it is regenerated only when one of those changes.
 """
@@ -83,8 +83,11 @@ def build_library() -> str:
 
 
 def build_cython() -> str:
-    """The Cython module `range_libc` (pywrapper/RangeLibc.pyx) over the C ABI,
-    linked against the library in this directory."""
+    """The Cython module over the C ABI (pywrapper/RangeLibc.pyx), linked
+    against the library in this directory.  It is built as `_range_libc`;
+    `import range_libc` goes through pywrapper/range_libc.py, which lets torch
+    (when installed) load its HIP runtime first, as _lib.py does, so that the
+    library binds to that runtime instead of bringing in a second one."""
     import sysconfig
 
     import numpy as np
@@ -92,9 +95,9 @@ def build_cython() -> str:
     pyx = os.path.join(PKG, "pywrapper", "RangeLibc.pyx")
     build_dir = os.path.join(PKG, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    c_file = os.path.join(build_dir, "range_libc.c")
-    out = os.path.join(PKG, "pywrapper", "range_libc" + sysconfig.get_config_var("EXT_SUFFIX"))
-    _run([sys.executable, "-m", "cython", "-3", "--module-name", "range_libc", "-o", c_file, pyx])
+    c_file = os.path.join(build_dir, "_range_libc.c")
+    out = os.path.join(PKG, "pywrapper", "_range_libc" + sysconfig.get_config_var("EXT_SUFFIX"))
+    _run([sys.executable, "-m", "cython", "-3", "--module-name", "_range_libc", "-o", c_file, pyx])
     tmp = out + ".tmp"
     _run(["gcc", "-O2", "-shared", "-fPIC", "-w", "-DNPY_NO_DEPRECATED_API=NPY_1_7_API_VERSION",
           "-I", sysconfig.get_paths()["include"], "-I", np.get_include(), "-I", os.path.join(ROOT, "include"),

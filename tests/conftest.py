@@ -16,6 +16,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long-running CPU check")
 
 
+@pytest.hookimpl(trylast=True)
+def pytest_collection_modifyitems(config, items):
+    """GPU tests hold inputs in torch tensors, and torch ships its own HIP
+    runtime beside the system one the library links.  torch's runtime has
+    been seen to find no device once the library's had opened it, while the
+    other order works, so torch initialises the device before any test
+    creates a context."""
+    if any(item.get_closest_marker("gpu") for item in items):
+        import torch
+
+        torch.cuda.is_available()
+
+
 def bits(a):
     """uint32 view for bit-exact float comparisons (NaN-safe)."""
     return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
