@@ -191,6 +191,21 @@ int rl_calc_range_repeat_angles_eval_sensor_model(rl_map* map, int method, float
                                                   const float* obs, double* weights,
                                                   int num_particles, int num_angles);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): the call above with
+ * the same arguments, results and errors, except that the poses, angles,
+ * observations and weights are staged in a static device array of
+ * RL_NOALLOC_SCRATCH_BYTES per GPU (each part rounded up to 256 bytes:
+ * 12*N + 8*M + 8*N bytes plus padding; about 200,000 particles at 60 angles)
+ * instead of buffers the context allocates.  Larger calls fail with RL_ERR_ARG.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (from two contexts, or two threads with two contexts) race. */
+#define RL_NOALLOC_SCRATCH_BYTES (4u << 20)
+int rl_calc_range_repeat_angles_eval_sensor_model_noalloc(rl_map* map, int method, float max_range,
+                                                          const float* ins, const float* angles,
+                                                          const float* obs, double* weights,
+                                                          int num_particles, int num_angles);
+
 /* --- batches, device pointers, stream-ordered -------------------------------- */
 int rl_calc_range_many_dev(rl_map* map, int method, float max_range, const float* d_ins,
                            float* d_outs, int n);

@@ -67,6 +67,8 @@ SIGNATURES = {
     "rl_eval_sensor_model": [_vp, _vp, _vp, _vp, C.c_int, C.c_int],
     "rl_calc_range_repeat_angles_eval_sensor_model": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, _vp,
                                                       C.c_int, C.c_int],
+    "rl_calc_range_repeat_angles_eval_sensor_model_noalloc": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, _vp,
+                                                              C.c_int, C.c_int],
     "rl_calc_range_many_dev": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
     "rl_numpy_calc_range_dev": [_vp, C.c_int, C.c_float, _vp, _vp, C.c_int],
     "rl_numpy_calc_range_angles_dev": [_vp, C.c_int, C.c_float, _vp, _vp, _vp, C.c_int, C.c_int],

@@ -38,6 +38,13 @@ using rl::MapDev;
 using rl::TrigDev;
 using rl::WorldDev;
 
+// Staging area of rl_calc_range_repeat_angles_eval_sensor_model_noalloc: one
+// copy of this array per device, held until the process ends.  Host code must
+// not pass the array's name to a kernel or a copy (on the host it is a stand-in,
+// not a device address): rl_create looks up its address on its device with
+// hipGetSymbolAddress and every use goes through that pointer (rl_ctx::d_scratch).
+static __device__ __align__(256) unsigned char scratch[RL_NOALLOC_SCRATCH_BYTES];
+
 // ---------------------------------------------------------------------------
 // Kernels
 
@@ -830,6 +837,7 @@ struct rl_ctx {
   // context is freed when the last of them is destroyed, in any order
   std::atomic<int> refs{1};
   std::mutex mu;
+  void* d_scratch = nullptr;  // this device's copy of `scratch` (no-allocation entry point)
   void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
 
@@ -1255,6 +1263,13 @@ int rl_create(int device, rl_ctx** out) {
     return fail(RL_ERR_HIP, "hipStreamCreate failed");
   }
   c->stream = c->own_stream;
+  // this device's address of the no-allocation staging array
+  if (hipGetSymbolAddress(&c->d_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !c->d_scratch) {
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return fail(RL_ERR_HIP, "hipGetSymbolAddress of the staging array failed");
+  }
   // One device buffer: sin bitmap | cos bitmap
   size_t b_s = tf->bits_s.size(), b_c = tf->bits_c.size();
   size_t total = b_s + b_c + 256;
@@ -1654,6 +1669,41 @@ int rl_calc_range_repeat_angles_eval_sensor_model(rl_map* m, int method, float m
   HIP_TRY(hipMemcpyAsync(d_obs, obs, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
   TRY(launch_fused(m, method, max_range, (const float*)d_ins, (const float*)d_ang,
                    (const float*)d_obs, (double*)d_w, N, M));
+  HIP_TRY(hipMemcpyAsync(weights, d_w, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RL_OK;
+}
+
+// The fused call above with its staging in the static `scratch` array instead
+// of the context's grown buffers: no device allocation.  Layout in the array,
+// each part 256-byte aligned: poses N x 3, angles M, observations M, weights N.
+static size_t noalloc_part(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+int rl_calc_range_repeat_angles_eval_sensor_model_noalloc(rl_map* m, int method, float max_range,
+                                                          const float* ins, const float* angles,
+                                                          const float* obs, double* weights, int N, int M) {
+  CHECK_ARG(m && ins && angles && obs && weights, "null argument");
+  CHECK_ARG(M >= 1 && N >= 0, "bad num_angles / num_particles");
+  const size_t b_ins = noalloc_part((size_t)N * 3 * sizeof(float));
+  const size_t b_ang = noalloc_part((size_t)M * sizeof(float));
+  const size_t b_w = noalloc_part((size_t)N * sizeof(double));
+  const size_t need = b_ins + 2 * b_ang + b_w;
+  if (need > (size_t)RL_NOALLOC_SCRATCH_BYTES)
+    return fail(RL_ERR_ARG, "%d particles x %d angles need %zu staging bytes, above the %u of the "
+                            "no-allocation entry point", N, M, need, (unsigned)RL_NOALLOC_SCRATCH_BYTES);
+  rl_ctx* c = m->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (N == 0) return RL_OK;
+  char* base = (char*)c->d_scratch;
+  float* d_ins = (float*)base;
+  float* d_ang = (float*)(base + b_ins);
+  float* d_obs = (float*)(base + b_ins + b_ang);
+  double* d_w = (double*)(base + b_ins + 2 * b_ang);
+  HIP_TRY(hipMemcpyAsync(d_ins, ins, (size_t)N * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_ang, angles, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(d_obs, obs, (size_t)M * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  TRY(launch_fused(m, method, max_range, d_ins, d_ang, d_obs, d_w, N, M));
   HIP_TRY(hipMemcpyAsync(weights, d_w, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return RL_OK;
