@@ -359,10 +359,12 @@ def bench_surrogate(args):
     """Secondary workload: the learned surrogate on the particle-filter step's
     map (2048x2048 corridors, max_range 500).  Trains on ``--sg-pairs`` rays
     marched on the GPU (uniform free-space origins and headings, minibatch
-    65536), then casts the step's 4000 x 60 rays through the checked start
-    point at several bound limits and compares them with the exact marcher:
-    error distribution, share of rays finished from the start point, map reads
-    per ray, and time per step against the plain marching kernel."""
+    16384; learning rate 1e-3, then 3e-4 and 1e-4 over the last half and
+    fifth of the pairs), then casts the step's 4000 x 60 rays through the
+    checked path at several bound limits and compares them with the exact
+    marcher: error distribution, share of rays finished from a certified start
+    point, map reads and dependent read rounds per ray, and time per step (at
+    the 0.5-cell bound) against the plain marching kernel."""
     import torch
 
     import range_libc_amd as rl
@@ -375,9 +377,11 @@ def bench_surrogate(args):
     img, ins, angles, table, true_pose, seed_noise = build_workload()
     m = rl.DeviceMap(ctx, np.ascontiguousarray(img.T.astype(np.uint8)))
     sg = rl.Surrogate(m, seed=0)
-    batch = 65536
+    batch = 16384
     t0 = time.perf_counter()
-    curve = sg.train(MAX_RANGE, n_pairs=args.sg_pairs, batch=batch, lr=1e-3, seed=1)
+    stages = ((1e-3, 0.5), (3e-4, 0.3), (1e-4, 0.2))
+    curve = np.concatenate([sg.train(MAX_RANGE, n_pairs=int(args.sg_pairs * frac), batch=batch, lr=lr, seed=1 + i)
+                            for i, (lr, frac) in enumerate(stages)])
     train_s = time.perf_counter() - t0
     n = PARTICLES * BEAMS
     d_ins = torch.from_numpy(ins).to(dev)
@@ -400,7 +404,8 @@ def bench_surrogate(args):
         q = np.quantile(err, [0.5, 0.9, 0.99])
         rows.append({"bound_max": bmax, "from_start": st["from_start"] / n,
                      "fallback_bound": st["fallback_bound"] / n, "fallback_check": st["fallback_check"] / n,
-                     "reads_per_ray": st["reads"] / n, "err_p50": q[0], "err_p90": q[1], "err_p99": q[2],
+                     "reads_per_ray": st["reads"] / n, "rounds_per_ray": st["rounds"] / n,
+                     "err_p50": q[0], "err_p90": q[1], "err_p99": q[2],
                      "err_max": float(err.max()), "exact_share": float(np.mean(err == 0.0))})
 
     def timed(fn, reps):
@@ -418,7 +423,7 @@ def bench_surrogate(args):
                                                            d_ang.data_ptr(), exact.data_ptr(), PARTICLES, BEAMS),
                      args.steps)
     sg_ms = timed(lambda: sg.calc_range_angles_dev(MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(),
-                                                   PARTICLES, BEAMS, bound_max=8.0, stats=False), args.steps)
+                                                   PARTICLES, BEAMS, bound_max=0.5, stats=False), args.steps)
     flops = 2.0 * n * (64 * 256 + 2 * 256 * 256 + 256 * 2)
     print(json.dumps({
         "metric": "surrogate rays/s relative to the plain GPU marcher (particle-filter step rays)",
@@ -428,7 +433,7 @@ def bench_surrogate(args):
         "train_pairs": int(curve.size * batch), "train_s": train_s,
         "train_pairs_per_s": curve.size * batch / train_s,
         "loss_first": float(np.mean(curve[:4])), "loss_last": float(np.mean(curve[-16:])),
-        "exact_reads_per_ray": exact_reads, "by_bound": rows, "timed_bound_max": 8.0,
+        "exact_reads_per_ray": exact_reads, "by_bound": rows, "timed_bound_max": 0.5,
         "config": {"workload": "surrogate_corridor2048_p4000_b60_r500", "batch": batch}}))
 
 
@@ -678,7 +683,7 @@ def main():
                     choices=["pf_step", "c1", "uneven64", "surrogate", "glt", "cddt", "bresenham"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "
                          "surrogate (learned start point on the step's map)")
-    ap.add_argument("--sg-pairs", type=int, default=763 * 65536, help="surrogate training pairs")
+    ap.add_argument("--sg-pairs", type=int, default=50_000_000, help="surrogate training pairs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()

@@ -299,9 +299,10 @@ int rl_surrogate_step(rl_surrogate* s, float lr);
 int rl_surrogate_train(rl_surrogate* s, float max_range, long long n_pairs, int batch, float lr, unsigned seed,
                        float* loss_curve);
 /* numpy_calc_range_angles through the checked surrogate (device pointers,
- * stream-ordered).  stats (host, 4 counters, optional; synchronises): rays
+ * stream-ordered).  stats (host, 5 counters, optional; synchronises): rays
  * marched from the origin because the bound exceeded bound_max, because the
- * start check failed, rays finished from the start point, distance-map reads. */
+ * start check failed, rays finished from the start point, distance-map reads,
+ * dependent read rounds (the path's latency in reads). */
 int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const float* d_ins,
                                        const float* d_angles, float* d_ranges, int num_particles,
                                        int num_angles, float bound_max, unsigned* stats);

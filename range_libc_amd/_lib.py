@@ -527,11 +527,11 @@ class Surrogate:
 
     def calc_range_angles_dev(self, max_range, d_ins, d_angles, d_ranges, num_particles, num_angles,
                               bound_max=0.5, stats=True):
-        st = (C.c_uint * 4)() if stats else None
+        st = (C.c_uint * 5)() if stats else None
         check(lib().rl_surrogate_calc_range_angles_dev(self.handle, max_range, _vp(d_ins), _vp(d_angles),
                                                        _vp(d_ranges), num_particles, num_angles, bound_max,
                                                        None if st is None else C.cast(st, C.POINTER(C.c_uint))))
         if st is None:
             return None
-        return dict(zip(("fallback_bound", "fallback_check", "from_start", "reads"), (int(v) for v in st)))
+        return dict(zip(("fallback_bound", "fallback_check", "from_start", "reads", "rounds"), (int(v) for v in st)))
 

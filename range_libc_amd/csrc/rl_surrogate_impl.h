@@ -17,13 +17,28 @@
 // pinball loss at quantile 0.95 between the bound and the observed absolute
 // error (the error is treated as a constant for this term).
 //
-// Checked inference: a ray whose bound is <= bound_max cells starts marching
-// at t0 = max(0, range - bound - 1) instead of 0 if the cell at t0 is free and
-// inside the map; the march from t0 must hit by range + bound + 2, otherwise
-// (and when the bound is larger) the ray is marched from 0.  A wall before t0
-// is not seen, and the samples from t0 need not land in the exact march's hit
-// cell, so the result approximates the exact march; its error is measured,
-// not bounded.
+// Checked inference: a ray whose bound b is <= bound_max cells and whose
+// range r lies in [0, max_range] is a candidate for finishing from the start
+// point t0 = max(0, r - b - 1):
+//  1. certificate that the march from the origin cannot hit before t0: the
+//     distance map is read at points of [0, t0] spaced at most
+//     RL_SG_CERT_STEP apart (independent reads); a point p of the ray lies in
+//     a cell less than |p - a| + sqrt(2) from the cell of sample a, so every
+//     cell the segment crosses is free when each pair of neighbouring samples
+//     a, b has (d_a - sqrt2)+ + (d_b - sqrt2)+ >= |b - a| (+ a rounding
+//     margin) and d > 0 at each sample;
+//  2. RL_SG_PROBES marches from t0 + k / RL_SG_PROBES (k = 0, 1, ...), with
+//     the reference's step rule, each until its first hit, stopped (and the
+//     ray rejected) past r + b + 2 or max_range.  The exact march reaches the
+//     wall with its own sample phase, which decides between neighbouring hit
+//     cells on oblique walls; the ray is accepted only when every probe hits
+//     the same cell, and its range is that cell's distance.
+// Rejected candidates and the other rays are marched from the origin (exact).
+// Accepted ranges can still differ from the exact march where its phase picks
+// a cell that no probe picked; the error is measured against the marcher.
+// Reads: every read of both steps and of the fallback march is counted, and
+// the dependent rounds of the path (one for the certificate, the longest
+// probe, the fallback march) separately.
 
 #define RL_SG_F 64    // Fourier features
 #define RL_SG_NF 32   // frequencies
@@ -538,17 +553,21 @@ __global__ void k_sg_rays_from_particles(MapDev m, TrigDev trig, const float* __
 }
 
 // Checked march from the surrogate's start point (see file comment).
-// out: n x 2 network outputs; stats[0..3] += fallbacks (bound too large),
-// fallbacks (check failed), rays marched from the start point, map reads.
+// out: n x 2 network outputs; stats[0..4] += fallbacks (bound too large),
+// fallbacks (check failed), rays finished from the start point, map reads,
+// dependent read rounds.
+#define RL_SG_CERT_STEP 8.0f  // largest gap between certificate samples (cells)
+#define RL_SG_PROBES 4
+#define RL_SG_PROBE_READS 64  // reads per probe before the ray is rejected
 __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_range,
                                                           const float4* __restrict__ rays,
                                                           const float* __restrict__ out, float bound_max,
                                                           float* __restrict__ ranges, float scale,
                                                           unsigned* __restrict__ stats, int n) {
   __shared__ float lut[RL_LUT_SIZE];
-  __shared__ unsigned cnt[4];
+  __shared__ unsigned cnt[5];
   for (int i = threadIdx.x; i < RL_LUT_SIZE; i += blockDim.x) lut[i] = m.steps[i];
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0u;
+  if (threadIdx.x < 5) cnt[threadIdx.x] = 0u;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -560,33 +579,83 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
     const float bound = expf(lb);
     const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
     float result = -1.0f;
-    int reads = 0;
+    int reads = 0, rounds = 0;
     bool tried = false;
     if (bound <= bound_max && pred >= 0.0f && pred <= max_range && rl::fast_march_ok(x0, y0, max_range)) {
       tried = true;
-      float t = pred - bound - 1.0f;
-      if (t < 0.0f) t = 0.0f;
+      float t0 = pred - bound - 1.0f;
+      if (t0 < 0.0f) t0 = 0.0f;
       const float t_end = pred + bound + 2.0f;
-      int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
-      if ((unsigned)px < W && (unsigned)py < H && rl::glb(m.edt)[(size_t)px * H + py] > 0.0f) {
-        while (t < max_range && t <= t_end) {
-          px = (int)(x0 + dx * t);
-          py = (int)(y0 + dy * t);
-          if ((unsigned)px >= W || (unsigned)py >= H) {
-            result = max_range;
-            break;
-          }
-          float d = rl::glb(m.edt)[(size_t)px * H + py];
-          ++reads;
-          if (d <= 0.0f) {
-            float xd = (float)px - x0, yd = (float)py - y0;
-            result = __builtin_sqrtf(xd * xd + yd * yd);
-            break;
-          }
-          float st = d * 0.999f;
-          t += (st < 1.0f) ? 1.0f : st;
+      // 1. certificate over [0, t0]: nc + 1 evenly spaced samples
+      bool ok = true;
+      const int nc = t0 > 0.0f ? (int)ceilf(t0 / RL_SG_CERT_STEP) : 0;
+      const float gap = nc > 0 ? t0 / (float)nc : 0.0f;
+      float dprev = 0.0f;
+      for (int k0 = 0; k0 <= nc; k0 += 16) {  // 16 independent reads in flight per round
+        float d[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int k = k0 + j;
+          const float tk = gap * (float)k;
+          const float fx = x0 + dx * tk, fy = y0 + dy * tk;
+          const int px = (int)fx, py = (int)fy;
+          d[j] = 0.0f;  // off the map or in (-1, 0), where truncation is not floor: no certificate
+          if (k <= nc && fx >= 0.0f && fy >= 0.0f && (unsigned)px < W && (unsigned)py < H)
+            d[j] = rl::glb(m.edt)[(size_t)px * H + py];
         }
-        if (result < 0.0f && t >= max_range) result = max_range;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int k = k0 + j;
+          const float cover = fmaxf(d[j] - 1.41422f, 0.0f) + fmaxf(dprev - 1.41422f, 0.0f);
+          if (k <= nc) {
+            if (!(d[j] > 0.0f) || (k > 0 && !(cover >= gap + 0.01f))) ok = false;
+            dprev = d[j];
+            ++reads;
+          }
+        }
+        ++rounds;
+      }
+      // 2. probes from t0 at RL_SG_PROBES phases, marched side by side
+      if (ok) {
+        float tp[RL_SG_PROBES], rp[RL_SG_PROBES];
+        int ep[RL_SG_PROBES];
+#pragma unroll
+        for (int k = 0; k < RL_SG_PROBES; ++k) {
+          tp[k] = t0 + (float)k * (1.0f / (float)RL_SG_PROBES);
+          ep[k] = 0;
+          rp[k] = 0.0f;
+        }
+        for (int step = 0; step < RL_SG_PROBE_READS; ++step) {
+          bool any = false;
+#pragma unroll
+          for (int k = 0; k < RL_SG_PROBES; ++k) {
+            // a probe past its window (r + b + 2), max_range or the map ends
+            // without a hit (1)
+            const int px = (int)(x0 + dx * tp[k]), py = (int)(y0 + dy * tp[k]);
+            const bool inb = (unsigned)px < W && (unsigned)py < H;
+            if (ep[k] == 0 && (!(tp[k] < max_range) || tp[k] > t_end || !inb)) ep[k] = 1;
+            if (ep[k] == 0) {
+              const float d = rl::glb(m.edt)[(size_t)px * H + py];
+              ++reads;
+              any = true;
+              if (d <= 0.0f) {
+                const float xd = (float)px - x0, yd = (float)py - y0;
+                rp[k] = __builtin_sqrtf(xd * xd + yd * yd);
+                ep[k] = 2;
+              } else {
+                const float st = d * 0.999f;
+                tp[k] += (st < 1.0f) ? 1.0f : st;
+              }
+            }
+          }
+          if (!any) break;
+          ++rounds;
+        }
+        bool agree = true;
+#pragma unroll
+        for (int k = 0; k < RL_SG_PROBES; ++k)
+          agree = agree && ep[k] == 2 && __float_as_uint(rp[k]) == __float_as_uint(rp[0]);
+        if (agree) result = rp[0];
       }
     }
     if (result < 0.0f) {
@@ -595,15 +664,17 @@ __global__ __launch_bounds__(256) void k_sg_checked_march(MapDev m, float max_ra
                    ? rl::rm_march_idx8<true>(m, lut, rl::Box{nullptr, 0, 0}, max_range, x0, y0, dx, dy, &c)
                    : rl::rm_march<true>(m.edt, m.width, m.height, max_range, x0, y0, dx, dy, &c);
       reads += c;
+      rounds += c;
       atomicAdd(&cnt[tried ? 1 : 0], 1u);
     } else {
       atomicAdd(&cnt[2], 1u);
     }
     atomicAdd(&cnt[3], (unsigned)reads);
+    atomicAdd(&cnt[4], (unsigned)rounds);
     ranges[i] = result * scale;
   }
   __syncthreads();
-  if (threadIdx.x < 4 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], cnt[threadIdx.x]);
+  if (threadIdx.x < 5 && cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], cnt[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -871,7 +942,7 @@ int rl_surrogate_create(rl_map* map, unsigned seed, rl_surrogate** out) {
   if ((rc = sg_alloc(&s->d_p, RL_SG_NPARAM)) || (rc = sg_alloc(&s->d_m1, RL_SG_NPARAM)) ||
       (rc = sg_alloc(&s->d_m2, RL_SG_NPARAM)) || (rc = sg_alloc(&s->d_g, RL_SG_NPARAM)) ||
       (rc = sg_alloc(&s->d_pb, RL_SG_NPARAM)) || (rc = sg_alloc(&s->d_freq, RL_SG_NF * 4)) ||
-      (rc = sg_alloc(&s->d_stats, 4))) {
+      (rc = sg_alloc(&s->d_stats, 5))) {
     sg_free(s);
     delete s;
     map->refs.fetch_sub(1);  // cannot reach zero: the caller holds the map
@@ -1056,8 +1127,8 @@ int rl_surrogate_train(rl_surrogate* s, float max_range, long long n_pairs, int 
 
 // Checked surrogate cast of a particle batch (numpy_calc_range_angles
 // semantics for inputs and outputs; ranges approximate, see file comment).
-// stats (host, 4 entries): fallbacks (bound), fallbacks (check), rays started
-// at the surrogate's range, distance-map reads.
+// stats (host, 5 entries): fallbacks (bound), fallbacks (check), rays finished
+// from the surrogate's start point, distance-map reads, dependent read rounds.
 int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const float* d_ins,
                                        const float* d_angles, float* d_ranges, int N, int M, float bound_max,
                                        unsigned* stats) {
@@ -1073,7 +1144,7 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
   hipStream_t st = c->stream;
   MapDev md = s->map->dev();
   if (stats) {  // a fill kernel, not a memset node, so that the launch sequence stays graph-safe
-    hipLaunchKernelGGL(k_sg_zero_u32, dim3(1), dim3(64), 0, st, s->d_stats, 4);
+    hipLaunchKernelGGL(k_sg_zero_u32, dim3(1), dim3(64), 0, st, s->d_stats, 5);
     HIP_TRY(hipGetLastError());
   }
   hipLaunchKernelGGL(k_sg_rays_from_particles, dim3(grid_for(n, 256)), dim3(256), 0, st, md, c->trig, d_ins,
@@ -1085,7 +1156,7 @@ int rl_surrogate_calc_range_angles_dev(rl_surrogate* s, float max_range, const f
                      s->d_stats, n);
   HIP_TRY(hipGetLastError());
   if (stats) {
-    HIP_TRY(hipMemcpyAsync(stats, s->d_stats, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(stats, s->d_stats, 5 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
   }
   return RL_OK;

@@ -212,10 +212,9 @@ def test_checked_inference_fallback_is_exact(trained):
 
 @pytest.mark.parametrize("bound_max", [0.5, 8.0])
 def test_checked_inference_error_and_reads(trained, bound_max):
-    """What the start point gains (distance-map reads per ray) and costs
-    (error against the exact march), printed.  A ray whose check fails pays
-    for both marches, so fewer reads in total is a property of a trained
-    network, asserted when a fifth of the rays finish from the start point."""
+    """Every ray against the exact march: the fallback rays bit for bit, the
+    rays finished from the start point within the stated error (p99 <= 0.5
+    cell), with the error distribution and reads printed."""
     exact, got, st, reads = _exact_and_checked(trained, bound_max=bound_max)
     n = exact.size
     assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == n
@@ -223,16 +222,153 @@ def test_checked_inference_error_and_reads(trained, bound_max):
     q = np.quantile(err, [0.5, 0.9, 0.99])
     print(f"from start {st['from_start'] / n:.3f}, fallback bound {st['fallback_bound'] / n:.3f}, "
           f"check {st['fallback_check'] / n:.3f}; err p50 {q[0]:.3g} p90 {q[1]:.3g} p99 {q[2]:.3g} "
-          f"max {err.max():.3g}; reads/ray {st['reads'] / n:.2f} vs exact {reads / n:.2f}")
+          f"max {err.max():.3g}; reads/ray {st['reads'] / n:.2f} vs exact {reads / n:.2f}; "
+          f"dependent rounds/ray {st['rounds'] / n:.2f}")
     # a hit cell's centre distance may exceed max_range by up to sqrt(2) cells
     assert np.all(np.isfinite(got)) and np.all(got >= 0.0) and np.all(got <= 120.0 + 1.5)
-    if st["from_start"] > n // 5:
-        assert st["reads"] < reads
+    assert np.quantile(err, 0.99) <= 0.5
+
+
+def _constant_prediction(sg, frac, log_bound):
+    """Network outputs forced to (frac, log_bound) for every ray."""
+    p, f = sg.params()
+    w4 = 256 * 64 + 256 + 2 * (256 * 256 + 256)
+    p[w4:w4 + 2 * 256] = 0.0
+    p[w4 + 2 * 256] = frac
+    p[w4 + 2 * 256 + 1] = log_bound
+    sg.set_params(p, f)
+
+
+def test_checked_inference_never_tunnels(ctx):
+    """A confident network that predicts 0.8 max_range for every ray: rays whose
+    exact march hits well before that start point must not finish from it (the
+    certificate over [0, t0] sees the wall) and come out exact; accepted rays
+    end, but for a few where the exact march's sample phase skips a corner the
+    probes hit, in the same cell or a neighbouring one."""
+    torch = _torch()
+    img = W.corridor_map(512, seed=5)
+    R = 200.0
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=3)
+    _constant_prediction(sg, 0.8, math.log(0.25))
+    n_part, n_beams = 400, 60
+    ins = W.world_poses(img, n_part, seed=4)
+    ang = W.beam_angles(n_beams)
+    d_ins = torch.from_numpy(ins).cuda()
+    d_ang = torch.from_numpy(ang).cuda()
+    got = torch.full((n_part * n_beams,), -1.0, dtype=torch.float32, device="cuda")
+    st = sg.calc_range_angles_dev(R, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(), n_part, n_beams,
+                                  bound_max=0.5)
+    g = got.cpu().numpy()
+    want = O.Oracle(img, R, "rm").numpy_calc_range_angles(ins, ang)
+    before = want < 0.8 * R - 0.25 - 1.0 - 2.0  # a hit at least two cells before the start point
+    assert before.sum() > 1000 and st["fallback_bound"] == 0
+    assert_bits_equal(g[before], want[before], "rays with a wall before the start point")
+    err = np.abs(g.astype(np.float64) - want)
+    print(f"from start {st['from_start']}, check failed {st['fallback_check']}, max err {err.max():.3g}")
+    assert st["from_start"] > 0 and np.quantile(err, 0.999) <= 1.5
+
+
+def _step_rays(img, ins, ang, R):
+    """Grid-frame rays (x0, y0, sin h, cos h) of numpy_calc_range_angles and the
+    exact ranges (the oracle)."""
+    rot = np.float32(-0.0 - 3 * math.pi / 2)
+    th = (-ins[:, 2] + rot).astype(np.float32)
+    hs = (th[:, None] - ang[None, :]).astype(np.float32).ravel()
+    m = ang.size
+    rays = np.stack([np.repeat(ins[:, 1], m), np.repeat(ins[:, 0], m), O.libc_sinf(hs), O.libc_cosf(hs)],
+                    1).astype(np.float32)
+    return rays, O.Oracle(img, R, "rm").numpy_calc_range_angles(ins, ang).astype(np.float64)
+
+
+def test_trained_surrogate_gains_over_untrained(ctx):
+    """What training gains: on a room with two columns (max_range 40), the
+    trained network's range error is a fraction of the untrained one's and its
+    error bound covers about 95% of the rays (the pinball quantile); through
+    the checked path every ray still matches the exact march to p99 <= 0.5
+    cell.  Reads and dependent rounds per ray are printed beside the exact
+    march's (short rays like these are cheap to march, so the path does not
+    save reads here)."""
+    torch = _torch()
+    img = np.zeros((128, 128), np.uint8)
+    img[:3, :] = img[-3:, :] = 1
+    img[:, :3] = img[:, -3:] = 1
+    img[40:48, 40:48] = img[80:88, 70:78] = 1
+    R = 40.0
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    ins = W.world_poses(img, 500, seed=8)
+    ang = W.beam_angles(60)
+    rays, exact = _step_rays(img, ins, ang, R)
+    sg = rl.Surrogate(dm, seed=5)
+    rng = np.random.default_rng(0)
+    f = np.empty((32, 4), np.float32)
+    f[:, :2] = rng.normal(0, 4.0, (32, 2))
+    f[:, 2:] = rng.normal(0, 1.0, (32, 2))
+    sg.set_params(freq=f)
+    e0 = np.abs(R * sg.forward(rays)[:, 0].astype(np.float64) - exact)
+    for i, lr in enumerate((1e-3, 3e-4, 1e-4)):
+        sg.train(R, n_pairs=4096 * 3000, batch=4096, lr=lr, seed=20 + i, curve=False)
+    out = sg.forward(rays)
+    e1 = np.abs(R * out[:, 0].astype(np.float64) - exact)
+    covered = float(np.mean(e1 <= np.exp(np.clip(out[:, 1], -20, 20))))
+    d_ins = torch.from_numpy(ins).cuda()
+    d_ang = torch.from_numpy(ang).cuda()
+    got = torch.full((exact.size,), -1.0, dtype=torch.float32, device="cuda")
+    cnt = torch.empty(exact.size, dtype=torch.int32, device="cuda")
+    dm.count_steps_dev(rl.RL_METHOD_RM, rl.RL_FRAME_ANGLES, R, d_ins.data_ptr(), d_ang.data_ptr(), cnt.data_ptr(),
+                       500, 60)
+    n = exact.size
+    for bmax in (0.5, 2.0):
+        st = sg.calc_range_angles_dev(R, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(), 500, 60, bound_max=bmax)
+        err = np.abs(got.cpu().numpy().astype(np.float64) - exact)
+        print(f"bound <= {bmax}: from start {st['from_start'] / n:.4f}, check failed {st['fallback_check'] / n:.4f}; "
+              f"reads/ray {st['reads'] / n:.2f}, rounds/ray {st['rounds'] / n:.2f} vs exact "
+              f"{int(cnt.sum().item()) / n:.2f}; err p99 {np.quantile(err, 0.99):.3g} max {err.max():.3g}")
+        assert np.quantile(err, 0.99) <= 0.5
+    print(f"network error p50 {np.median(e0):.2f} -> {np.median(e1):.2f} cells, bound covers {covered:.3f}")
+    assert np.median(e1) < 0.25 * np.median(e0)
+    assert 0.9 <= covered <= 0.99
+
+
+def test_surrogate_full_size_corridor(ctx):
+    """The learned form at its configured size: a network trained on 50 M rays
+    marched on the GPU over the 2048^2 corridor map (max_range 500), then the
+    4000-particle x 60-beam step through the checked path at the 0.5-cell
+    bound and at 8 cells.  Every ray is compared with the oracle: the rays not
+    finished from a start point are exact, so at least that many match bit for
+    bit, and the p99 error over all rays is at most 0.5 cell.  The share of
+    rays that qualify is printed (on this map the trained bound stays above a
+    few cells, so at 0.5 none do)."""
+    torch = _torch()
+    img = W.corridor_map(2048, 1)
+    R = 500.0
+    ins = W.world_poses(img, 4000, 2)
+    ang = W.beam_angles(60)
+    want = O.Oracle(img, R, "rm").numpy_calc_range_angles(ins, ang)
+    dm = rl.DeviceMap(ctx, O.xmajor(img))
+    sg = rl.Surrogate(dm, seed=0)
+    for i, (lr, pairs) in enumerate(((1e-3, 25_000_000), (3e-4, 15_000_000), (1e-4, 10_000_000))):
+        sg.train(R, n_pairs=pairs, batch=16384, lr=lr, seed=10 + i, curve=False)
+    d_ins = torch.from_numpy(ins).cuda()
+    d_ang = torch.from_numpy(ang).cuda()
+    got = torch.full((want.size,), -1.0, dtype=torch.float32, device="cuda")
+    n = want.size
+    for bmax in (0.5, 8.0):
+        st = sg.calc_range_angles_dev(R, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(), 4000, 60, bound_max=bmax)
+        g = got.cpu().numpy()
+        err = np.abs(g.astype(np.float64) - want)
+        q = np.quantile(err, [0.5, 0.99, 0.999])
+        print(f"bound <= {bmax}: from start {st['from_start'] / n:.4f}, check failed {st['fallback_check'] / n:.4f}, "
+              f"bound too large {st['fallback_bound'] / n:.4f}; err p50 {q[0]:.3g} p99 {q[1]:.3g} p99.9 {q[2]:.3g} "
+              f"max {err.max():.3g}; reads/ray {st['reads'] / n:.2f}, rounds/ray {st['rounds'] / n:.2f}")
+        assert st["fallback_bound"] + st["fallback_check"] + st["from_start"] == n
+        assert np.sum(g.view(np.uint32) == want.view(np.uint32)) >= n - st["from_start"]
+        assert q[1] <= 0.5
 
 
 def test_checked_inference_untrained_any_bound(ctx):
-    """An untrained network with an unlimited bound: every ray tries the start
-    point; results stay within [0, max_range] and the counters add up."""
+    """An untrained network with an unlimited bound: every ray is a candidate;
+    results stay within [0, max_range] and the counters add up."""
     torch = _torch()
     img = W.corridor_map(300, seed=3)
     dm = rl.DeviceMap(ctx, O.xmajor(img))
