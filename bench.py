@@ -149,6 +149,112 @@ def _kernel_ms(torch, stream, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def dependent_load_ns(torch, ctx, dev, steps=20000):
+    """ns per dependent load of one lane chasing a random cycle, one element per
+    128-byte line: through 2 MiB after a full warm pass (held in one XCD's
+    4 MiB L2), through the first lines of a 64 MiB cycle after a full warm pass
+    (evicted from L2 by the rest of the pass, held in the 256 MiB Infinity
+    Cache), and through 2 GiB never read before (HBM).  Timed with HIP events
+    around one launch of `steps` loads on the library's stream (launch
+    overhead below 0.1%)."""
+    out = {}
+    rng = np.random.default_rng(0)
+    res = torch.zeros(2, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    for name, size, warm in (("l2_2MiB", 2 << 20, True), ("mall_64MiB", 64 << 20, True), ("hbm_2GiB", 2 << 30, False)):
+        n = size // 128
+        perm = rng.permutation(n).astype(np.int64)
+        nxt = np.zeros(size // 4, np.uint32)
+        nxt[perm * 32] = (np.roll(perm, -1) * 32).astype(np.uint32)
+        del perm
+        d = torch.from_numpy(nxt).to(dev)
+        del nxt
+        ctx.debug_chase(d.data_ptr(), 1, res.data_ptr())  # code and first line loaded
+        if warm:
+            ctx.debug_chase(d.data_ptr(), max(n, steps), res.data_ptr())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.debug_chase(d.data_ptr(), steps, res.data_ptr())
+        e1.record(stream)
+        stream.synchronize()
+        out[name] = e0.elapsed_time(e1) * 1e6 / steps
+        del d
+        torch.cuda.empty_cache()
+    return out
+
+
+def _march_seq(edt, x0, y0, dx, dy, max_range):
+    """Distances read by the march of one ray, in order (float32 as the kernel)."""
+    f32, one, c999 = np.float32, np.float32(1.0), np.float32(0.999)
+    x0, y0, dx, dy = f32(x0), f32(y0), f32(dx), f32(dy)
+    seq = []
+    t = f32(0.0)
+    while t < max_range:
+        px, py = int(x0 + dx * t), int(y0 + dy * t)
+        if not (0 <= px < edt.shape[0] and 0 <= py < edt.shape[1]):
+            break
+        d = edt[px, py]
+        seq.append(d)
+        if d <= 0:
+            break
+        st = d * c999
+        t = f32(t + (one if st < one else st))
+    return seq
+
+
+_TABULATED = None
+
+
+def _rounds(seq, start_k=0, prev=None, width=2, limit=None):
+    """Rounds of the march schedule over a read sequence from position start_k:
+    one read per round for the first 8, then speculation of `width` positions
+    (a round also resolves the positions after cells that repeat the previous
+    tabulated distance).  Stops after `limit` rounds; returns (rounds, k, prev)."""
+    global _TABULATED
+    if _TABULATED is None:
+        sq = sorted({a * a + b * b for a in range(32) for b in range(a, 32) if a * a + b * b < 1024})[:255]
+        _TABULATED = {np.float32(np.sqrt(np.float32(v))) for v in sq}
+    rounds, k = 0, start_k
+    while k < len(seq) and (limit is None or rounds < limit):
+        if k < 8 or prev is None or prev not in _TABULATED:
+            k += 1
+        else:
+            j = 0
+            while j < width - 1 and k + j < len(seq) and seq[k + j] == prev:
+                j += 1
+            k = min(k + j + 1, len(seq))
+        prev = seq[k - 1]
+        rounds += 1
+    return rounds, k, prev
+
+
+def longest_chain_rounds(m, ins, angles, counts, ctx, torch, dev, top=16):
+    """Dependent read rounds of the longest waves under the particle kernel's
+    schedule, replayed from the device's own distance map and sinf/cosf: every
+    ray of the wave takes width-2 rounds; once one ray is left, the wave takes
+    it on with 8-wide speculation.  Returns the most rounds among the `top`
+    waves whose rays read the most cells."""
+    edt = m.edt()
+    M = angles.size
+    parts = np.argsort(counts.reshape(-1, M).max(1))[::-1][:top]
+    rot = np.float32(-0.0 - 3 * np.pi / 2)
+    heads = ((-ins[parts, 2] + rot).astype(np.float32)[:, None] - angles[None, :]).astype(np.float32).ravel()
+    dh = torch.from_numpy(heads).to(dev)
+    ds, dc = torch.empty_like(dh), torch.empty_like(dh)
+    ctx.sincosf_dev(dh.data_ptr(), ds.data_ptr(), dc.data_ptr(), heads.size)
+    torch.cuda.synchronize()
+    sn, cs = ds.cpu().numpy().reshape(-1, M), dc.cpu().numpy().reshape(-1, M)
+    best = 0
+    for i, p in enumerate(parts):
+        seqs = [_march_seq(edt, ins[p, 1], ins[p, 0], cs[i, a], sn[i, a], MAX_RANGE) for a in range(M)]
+        r = sorted((_rounds(q)[0], a) for a, q in enumerate(seqs))
+        r2, (_, a1) = (r[-2][0] if len(r) > 1 else 0), r[-1]
+        n1, k, prev = _rounds(seqs[a1], limit=r2)
+        n2, _, _ = _rounds(seqs[a1], start_k=k, prev=prev, width=8)
+        best = max(best, n1 + n2)
+    return best
+
+
 def bench_c1(args):
     """The reference's CPU-runnable configuration: plain ray marching through
     numpy_calc_range (RangeLib.h:439-480, 927-962) on a seeded 512x512 map of 25
@@ -570,6 +676,17 @@ def bench_bresenham(args):
                      "kernel": "k_cast<BL, world>", "alg_bytes_per_launch": alg["bl"],
                      "visited_cells_per_s": cells["bl"] / (ms["bl"] * 1e-3)},
         "config": {"workload": "bresenham_vs_rm_corridor2048_1m_r1000"}}
+    # the instruction-issue bounds, from the committed counter pass over this launch
+    ipath = os.path.join(ROOT, "profiles", "bresenham_issue_corridor2048_1m.json")
+    if os.path.exists(ipath):
+        with open(ipath) as f:
+            ij = json.load(f)["bresenham"]
+        result["issue_bound"] = {
+            "valu_us": ij["valu_issue_bound_us"], "salu_us": ij["salu_issue_bound_us"],
+            "valu_busy_frac_at_profiled_duration": ij["valu_busy_frac"],
+            "valu_busy_frac_here": ij["valu_issue_bound_us"] / (ms["bl"] * 1e3),
+            "source": "profiles/bresenham_issue_corridor2048_1m.json: rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU "
+                      "GRBM_GUI_ACTIVE over this launch; not measured in this run"}
     from oracle import oracle as O
 
     pick = np.random.default_rng(0).choice(NQ, 4000, replace=False)
@@ -863,11 +980,25 @@ def main():
         host[name] = (time.perf_counter() - t0) * 1e3 / reps
     assert np.array_equal(h_w, w_eager) and np.array_equal(h_rng, r_eager), "host API differs from the device step"
 
-    traffic = None
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_rm_angles_pf_step.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_src = (f"profiles/traffic_rm_angles_pf_step.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                       f"passes over this kernel (profiled launches {tj.get('profiled_launch_us', '?')} us), "
+                       f"scaled by a 256 MiB copy's counts; not measured in this run")
+
+    # the longest dependent chain against the measured latency of one dependent load
+    lat = dependent_load_ns(torch, ctx, dev)
+    cnt_h = d_cnt.cpu().numpy()
+    chain_rounds = longest_chain_rounds(m, ins, angles, cnt_h, ctx, torch, dev)
+    chain = {"max_reads_per_ray": int(cnt_h.max()), "longest_chain_rounds": chain_rounds,
+             "dependent_load_ns": lat,
+             "chain_bound_us_l2": chain_rounds * lat["l2_2MiB"] / 1e3,
+             "chain_bound_us_mall": chain_rounds * lat["mall_64MiB"] / 1e3,
+             "chain_bound_us_hbm": chain_rounds * lat["hbm_2GiB"] / 1e3}
 
     result = {
         "metric": "rays/s on 1 GPU (particle-filter step: ray marching over repeated angles + sensor model)",
@@ -886,10 +1017,11 @@ def main():
                    "particles": PARTICLES, "beams": BEAMS, "max_range": MAX_RANGE, "sensor_table": TABLE,
                    "parallelism": "single_gpu", "xcd_scheduling": bool(args.xcd), "lds_box": bool(args.box)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_particles<RM, sensor> (rl_pf_step_dev)", "kernel_ms": kern_ms,
                      "kernel_ms_event_pair_per_launch": kern_pair_ms, "kernel_launches_timed": KREPS,
                      "alg_bytes_per_launch": alg_bytes, "edt_reads_per_ray": reads / rays},
+        "chain": chain,
         "eager_ms_per_step": eager_ms,
         "two_launch_eager_ms_per_step": two_launch_ms,
         "graph_group": GROUP,

@@ -317,6 +317,11 @@ int rl_count_steps_dev(rl_map* map, int method, int frame, float max_range, cons
 int rl_debug_sincosf_dev(rl_ctx* ctx, const float* d_x, float* d_sin, float* d_cos, int n);
 /* Device sqrtf as the kernels use it. */
 int rl_debug_sqrtf_dev(rl_ctx* ctx, const float* d_x, float* d_out, int n);
+/* Dependent-load latency probe (no reference counterpart): one lane loads
+ * i = d_next[i] `steps` times from i = 0; d_out[0] = elapsed wall-clock ticks,
+ * d_out[1] = the final index; *tick_ns (host, optional) = ns per tick.  Every
+ * d_next entry must index inside the buffer. */
+int rl_debug_chase_dev(rl_ctx* ctx, const unsigned* d_next, int steps, long long* d_out, double* tick_ns);
 /* C = A * B through the surrogate's bf16 matrix-core GEMM (host row-major
  * matrices of bf16 bit patterns; fp32 result). */
 int rl_debug_gemm_bf16(rl_ctx* ctx, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K);

@@ -86,6 +86,7 @@ SIGNATURES = {
     "rl_batch_launch": [_vp],
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
+    "rl_debug_chase_dev": [_vp, _vp, C.c_int, _vp, C.POINTER(C.c_double)],
     "rl_debug_gemm_bf16": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "rl_debug_gemm_bf16_nt": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "rl_surrogate_create": [_vp, C.c_uint, C.POINTER(_vp)],
@@ -189,6 +190,12 @@ class Context:
 
     def sqrtf_dev(self, x_ptr: int, out_ptr: int, n: int):
         check(lib().rl_debug_sqrtf_dev(self.handle, _vp(x_ptr), _vp(out_ptr), n))
+
+    def debug_chase(self, next_ptr: int, steps: int, out_ptr: int) -> float:
+        """Dependent-load latency probe; returns ns per wall-clock tick."""
+        tick = C.c_double()
+        check(lib().rl_debug_chase_dev(self.handle, _vp(next_ptr), steps, _vp(out_ptr), C.byref(tick)))
+        return tick.value
 
 
 _DEFAULT: dict[int, Context] = {}

@@ -221,11 +221,20 @@ __device__ __forceinline__ float idx8_step(const MapDev& m, const float* lut, un
 // stops at the first mismatch and the next round starts from the true next t.
 // Positions, float operations and results are those of the plain loop, and
 // `reads` counts resolved positions only.
-template <bool COUNT, bool BOX>
+// State a lane hands to solo_tail when it is the last of its wave still
+// marching (see there): its ray and its position, last step and that step's
+// code.  pending = false: the lane's range is final.
+struct MarchTail {
+  bool pending;
+  float x0, y0, dx, dy, t, g;
+  unsigned gcode;
+};
+
+template <bool COUNT, bool BOX, bool SOLO = false>
 __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* lut, const Box& box,
                                                  float max_range, float x0, float y0, float dx,
                                                  float dy, int* reads, float t, float g,
-                                                 unsigned gcode, int nr) {
+                                                 unsigned gcode, int nr, MarchTail* tail = nullptr) {
   // (t, g, gcode, nr): position, last step, the code that gave it (256 matches
   // no cell) and reads so far; (0, 1, 1, 0) for a march from its origin
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
@@ -246,6 +255,11 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     t += g;
   }
   while (RL_SPEC == 2 && !end) {  // width 2, written out
+    if (SOLO && __popcll(__builtin_amdgcn_read_exec()) == 1) {
+      // the wave's last marching lane: the whole wave takes over (solo_tail)
+      *tail = MarchTail{true, x0, y0, dx, dy, t, g, gcode};
+      return 0.0f;
+    }
     const float t0 = t, t1 = t + g;
     const int px0 = (int)(x0 + dx * t0), py0 = (int)(y0 + dy * t0);
     const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
@@ -361,10 +375,10 @@ __device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut
 // load overlaps the sinf/cosf evaluation.  A NaN direction (NaN or infinite
 // heading) returns max_range, as in the reference, whose float-to-int
 // conversion of a NaN position gives INT_MIN, a cell outside the map.
-template <bool COUNT, bool BOX>
+template <bool COUNT, bool BOX, bool SOLO = false>
 __device__ __forceinline__ float rm_cast_idx8_t(const MapDev& m, const float* lut, const Box& box,
                                                 const TrigDev& trig, float max_range, float x0, float y0,
-                                                float heading, int* reads) {
+                                                float heading, int* reads, MarchTail* tail = nullptr) {
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
   const int px = (int)x0, py = (int)y0;
   const bool in = (unsigned)px < W && (unsigned)py < H;
@@ -380,8 +394,77 @@ __device__ __forceinline__ float rm_cast_idx8_t(const MapDev& m, const float* lu
     return __builtin_sqrtf(xd * xd + yd * yd);
   }
   const float g = idx8_step(m, lut, c, px, py);
-  return rm_march_idx8_t<COUNT, BOX>(m, lut, box, max_range, x0, y0, dx, dy, reads, g, g,
-                                     (c != RL_IDX8_ESCAPE) ? c : 256u, 1);
+  return rm_march_idx8_t<COUNT, BOX, SOLO>(m, lut, box, max_range, x0, y0, dx, dy, reads, g, g,
+                                           (c != RL_IDX8_ESCAPE) ? c : 256u, 1, tail);
+}
+
+// The end of a march whose lane was left alone in its wave (MarchTail): the
+// whole wave, converged, follows that one ray with lane-parallel speculation.
+// Lane k (of RL_SOLO_W) forms t_k = t + g + ... + g by k float additions in
+// order -- the positions the march reaches if its next k steps all repeat the
+// last step g -- and loads its cell.  The first lane whose cell ends the ray
+// or carries another code than g's is the next true position: positions up
+// to it were read and continued with step g, and the march goes on from its
+// cell as rm_march_idx8_t does (same float operations, same bits).  When all
+// RL_SOLO_W cells repeat g's code, the march continues from the last of them.
+// Returns the ray's range on the pending lane and `d` on the others.  Must be
+// reached by the whole wave.
+#ifndef RL_SOLO_W
+#define RL_SOLO_W 8
+#endif
+#ifndef RL_SOLO_TAIL
+#define RL_SOLO_TAIL 1  // 0: every lane finishes its own march
+#endif
+__device__ __forceinline__ float solo_tail(const MapDev& m, const float* lut, float max_range,
+                                           const MarchTail& tail, float d) {
+  unsigned long long pm = __builtin_amdgcn_ballot_w64(tail.pending);
+  auto rdf = [](float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+  for (; pm; pm &= pm - 1ull) {  // one pending lane per wave in practice
+  const int L = __builtin_ctzll(pm);
+  const float x0 = rdf(tail.x0, L), y0 = rdf(tail.y0, L), dx = rdf(tail.dx, L), dy = rdf(tail.dy, L);
+  float t = rdf(tail.t, L), g = rdf(tail.g, L);
+  unsigned gcode = (unsigned)__builtin_amdgcn_readlane((int)tail.gcode, L);
+  const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
+  const int k = (int)(threadIdx.x & (RL_SOLO_W - 1));  // lanes beyond RL_SOLO_W repeat the first ones
+  int end, hx, hy;
+  for (;;) {
+    float tk = t;
+#pragma unroll
+    for (int j = 0; j < RL_SOLO_W - 1; ++j)
+      if (j < k) tk = tk + g;
+    const int px = (int)(x0 + dx * tk), py = (int)(y0 + dy * tk);
+    const bool live = tk < max_range;
+    const bool inb = (unsigned)px < W && (unsigned)py < H;
+    const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, inb ? (unsigned)px : 0u, inb ? (unsigned)py : 0u);
+    const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
+    const unsigned long long stop =
+        __builtin_amdgcn_ballot_w64(e != 0 || c != gcode) & ((1ull << RL_SOLO_W) - 1ull);
+    const int j = stop ? __builtin_ctzll(stop) : RL_SOLO_W - 1;
+    const float tj = rdf(tk, j);
+    if (!stop) {  // every cell repeated g's code: RL_SOLO_W positions continued with step g
+      t = tj + g;
+      continue;
+    }
+    const int ej = __builtin_amdgcn_readlane(e, j);
+    const int pxj = __builtin_amdgcn_readlane(px, j), pyj = __builtin_amdgcn_readlane(py, j);
+    if (ej) {
+      end = ej, hx = pxj, hy = pyj;
+      break;
+    }
+    const unsigned cj = (unsigned)__builtin_amdgcn_readlane((int)c, j);
+    g = idx8_step(m, lut, cj, pxj, pyj);
+    gcode = (cj != RL_IDX8_ESCAPE) ? cj : 256u;
+    t = tj + g;
+  }
+  float r = max_range;
+  if (end == 2) {
+    const float xd = (float)hx - x0;
+    const float yd = (float)hy - y0;
+    r = __builtin_sqrtf(xd * xd + yd * yd);
+  }
+  if ((int)(threadIdx.x & 63) == L) d = r;
+  }
+  return d;
 }
 
 template <bool COUNT>

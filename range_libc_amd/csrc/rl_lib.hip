@@ -528,16 +528,28 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
   rl::Box box{nullptr, 0, 0};
   if (METHOD == RL_METHOD_RM && S == 64 && m.idx8 && m.use_box && box_lds)
     box = rl::stage_box(m, box_lds, gy, gx, valid && rl::fast_march_ok(gy, gx, max_range));
+  // ray marching over the compact map: a wave's last marching ray is finished
+  // by the whole wave (rl::solo_tail); the same for every lane of the wave
+  const bool solo = RL_SOLO_TAIL && METHOD == RL_METHOD_RM && m.idx8 && !box.lds;
+  const bool fast = rl::fast_march_ok(gy, gx, max_range);
   double w = 1.0;
   for (int c0 = 0; c0 < M; c0 += S) {
     int j = c0 + j0;
     double term = 1.0;
-    if (valid && j < M) {
+    const bool act = valid && j < M;
+    const bool first = c0 == 0 && pre.ok;
+    rl::MarchTail tail{false, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u};
+    float d = 0.0f;
+    if (act) {
       int unused;
-      const bool first = c0 == 0 && pre.ok;
       const float aj = first ? pre.a0 : angles[j];
-      float d = cast_one<METHOD, false>(m, lut, box, trig, max_range, gy, gx, theta - aj, &unused) *
-                m.world.scale;
+      d = (solo && fast) ? rl::rm_cast_idx8_t<false, false, true>(m, lut, box, trig, max_range, gy, gx, theta - aj,
+                                                                  &unused, &tail)
+                         : cast_one<METHOD, false>(m, lut, box, trig, max_range, gy, gx, theta - aj, &unused);
+    }
+    if (solo) d = rl::solo_tail(m, lut, max_range, tail, d);
+    if (act) {
+      d = d * m.world.scale;
       ranges[(size_t)p * M + j] = d;
       if (SENSOR) {
         float r = rl::clamp_table((first ? pre.o0 : obs[j]) * m.world.inv_scale, hi);
@@ -692,6 +704,22 @@ __global__ void k_debug_sincos(TrigDev trig, const float* __restrict__ x, float*
 __global__ void k_debug_sqrt(const float* __restrict__ x, float* __restrict__ out, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = __builtin_sqrtf(x[i]);
+}
+
+// Dependent-load latency: one lane follows next[] for `steps` loads (each
+// address is the previous load's value) and records the elapsed wall-clock
+// ticks and the final index (so the loads cannot be dropped).  The caller
+// bounds every next[] entry by the buffer's length.
+__global__ void k_debug_chase(const unsigned* __restrict__ next, int steps, long long* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  unsigned i = 0;
+  const long long t0 = wall_clock64();
+  for (int k = 0; k < steps; ++k) i = rl::glb(next)[i];
+  // the last load completes before the second clock read
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::"v"(i) : "memory");
+  const long long t1 = wall_clock64();
+  out[0] = t1 - t0;
+  out[1] = (long long)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1806,6 +1834,20 @@ int rl_debug_sqrtf_dev(rl_ctx* c, const float* x, float* out, int n) {
   TRY(use_device(c));
   if (n == 0) return RL_OK;
   hipLaunchKernelGGL(k_debug_sqrt, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, x, out, n);
+  HIP_TRY(hipGetLastError());
+  return RL_OK;
+}
+
+int rl_debug_chase_dev(rl_ctx* c, const unsigned* next, int steps, long long* out, double* tick_ns) {
+  CHECK_ARG(c && next && out && steps >= 0, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  if (tick_ns) {
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    *tick_ns = khz > 0 ? 1e6 / (double)khz : 0.0;
+  }
+  hipLaunchKernelGGL(k_debug_chase, dim3(1), dim3(64), 0, c->stream, next, steps, out);
   HIP_TRY(hipGetLastError());
   return RL_OK;
 }

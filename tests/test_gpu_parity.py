@@ -455,3 +455,24 @@ def test_sensor_model_nan_observations_and_ranges(corridor_map):
     wf = np.empty(n, np.float64)
     meth.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs, wf)
     assert np.array_equal(wf, orc.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))
+
+
+def test_dependent_load_probe(ctx):
+    """The latency probe follows the chain it is given: the final index equals
+    the host's walk of the same cycle, and the elapsed time is positive."""
+    torch = _torch()
+    rng = np.random.default_rng(3)
+    n = 4096
+    perm = rng.permutation(n)
+    nxt = np.zeros(n * 32, np.uint32)
+    nxt[perm * 32] = (np.roll(perm, -1) * 32).astype(np.uint32)
+    d = torch.from_numpy(nxt).cuda()
+    out = torch.zeros(2, dtype=torch.int64, device="cuda")
+    steps = 1000
+    tick_ns = ctx.debug_chase(d.data_ptr(), steps, out.data_ptr())
+    torch.cuda.synchronize()
+    i = 0
+    for _ in range(steps):
+        i = int(nxt[i])
+    assert int(out[1].item()) == i
+    assert int(out[0].item()) > 0 and tick_ns > 0
