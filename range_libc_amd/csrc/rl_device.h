@@ -45,7 +45,9 @@ struct WorldDev {
 // (d <= 27.3) are tabulated once (lut); the map stores per cell the index of
 // its value in that table, or RL_IDX8_ESCAPE when the value is not tabulated
 // (the fp32 map is then read).  Cells are stored in 8x8 blocks of 64 bytes,
-// blocks x-major: byte (((x >> 3) * nby + (y >> 3)) << 6) | ((x & 7) << 3) | (y & 7).
+// blocks x-major, y-major inside a block: byte (((x >> 3) * nby + (y >> 3)) << 6) |
+// ((y & 7) << 3) | (x & 7), which is (x >> 3) * 64 * nby + 8 * y + (x & 7): one
+// shift, one mask, one shift-add and one multiply-add per cell address.
 #define RL_IDX8_ESCAPE 255u
 #define RL_LUT_SIZE 256
 
@@ -178,14 +180,15 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
 // the box covers the cell, else from the map in HBM.
 template <bool BOX>
 __device__ __forceinline__ unsigned idx8_code(const MapDev& m, const Box& box, unsigned ux, unsigned uy) {
-  const unsigned inner = ((ux & 7u) << 3) | (uy & 7u);
   if (BOX) {
     unsigned ib = (unsigned)((int)(ux >> 3) - box.bx0), jb = (unsigned)((int)(uy >> 3) - box.by0);
-    if (ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS) return lds(box.lds)[((ib * RL_BOX_BLOCKS + jb) << 6) | inner];
+    if (ib < RL_BOX_BLOCKS && jb < RL_BOX_BLOCKS)
+      return lds(box.lds)[((ib * RL_BOX_BLOCKS + jb) << 6) | ((uy & 7u) << 3) | (ux & 7u)];
   }
   // 32-bit offset from the map's base (the compact map is built only below
-  // 4 GiB): one multiply-add and a scalar-base load on the march's chain
-  return glb(m.idx8)[((__umul24(ux >> 3, (unsigned)m.nby) + (uy >> 3)) << 6) | inner];
+  // 4 GiB, so 64 * nby < 2^24): a multiply-add on the march's chain and a
+  // scalar-base load
+  return glb(m.idx8)[__umul24(ux >> 3, (unsigned)m.nby << 6) + ((uy << 3) | (ux & 7u))];
 }
 
 // March step max(d * 0.999f, 1.0f) from a cell with non-zero code c: from the

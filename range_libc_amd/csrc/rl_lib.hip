@@ -672,8 +672,8 @@ __global__ void k_build_idx8(const float* __restrict__ edt, const float* __restr
   if (i >= nbytes) return;
   size_t b = i >> 6;
   unsigned o = (unsigned)(i & 63);
-  unsigned px = (unsigned)(b / (size_t)nby) * 8u + (o >> 3);
-  unsigned py = (unsigned)(b % (size_t)nby) * 8u + (o & 7u);
+  unsigned px = (unsigned)(b / (size_t)nby) * 8u + (o & 7u);
+  unsigned py = (unsigned)(b % (size_t)nby) * 8u + (o >> 3);
   unsigned code = RL_IDX8_ESCAPE;
   if (px < (unsigned)W && py < (unsigned)H) {
     float d = edt[(size_t)px * H + py];
