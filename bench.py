@@ -322,7 +322,7 @@ def bench_c1(args):
                                                              f"numpy_calc_range over all {N} queries on 1 thread")
         te = timed_runs(lambda: O.ref_edt(img), 2.0, min_runs=5, warmup=1)
         result["cpu_baseline"]["edt_build_ms"] = float(np.median(te)) * 1e3
-    print(json.dumps(result))
+    return result
 
 
 def bench_uneven64(args):
@@ -419,7 +419,7 @@ def bench_uneven64(args):
         "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}
     if not args.no_cpu:
         result["cpu_baseline"] = uneven64_cpu_baseline(specs, table, keep, args.cpu_budget)
-    print(json.dumps(result))
+    return result
 
 
 def uneven64_cpu_baseline(specs, table, keep, budget_s):
@@ -531,7 +531,7 @@ def bench_surrogate(args):
     sg_ms = timed(lambda: sg.calc_range_angles_dev(MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), got.data_ptr(),
                                                    PARTICLES, BEAMS, bound_max=0.5, stats=False), args.steps)
     flops = 2.0 * n * (64 * 256 + 2 * 256 * 256 + 256 * 2)
-    print(json.dumps({
+    return {
         "metric": "surrogate rays/s relative to the plain GPU marcher (particle-filter step rays)",
         "value": plain_ms / sg_ms, "unit": "x", "surrogate_ms_per_step": sg_ms, "plain_ms_per_step": plain_ms,
         "surrogate_rays_per_s": n / (sg_ms * 1e-3), "plain_rays_per_s": n / (plain_ms * 1e-3),
@@ -540,7 +540,7 @@ def bench_surrogate(args):
         "train_pairs_per_s": curve.size * batch / train_s,
         "loss_first": float(np.mean(curve[:4])), "loss_last": float(np.mean(curve[-16:])),
         "exact_reads_per_ray": exact_reads, "by_bound": rows, "timed_bound_max": 0.5,
-        "config": {"workload": "surrogate_corridor2048_p4000_b60_r500", "batch": batch}}))
+        "config": {"workload": "surrogate_corridor2048_p4000_b60_r500", "batch": batch}}
 
 
 def bench_glt(args):
@@ -614,7 +614,7 @@ def bench_glt(args):
                                       "sample": f"GiantLUTCast constructor on a 256x256 corridor map, {TD} bins: "
                                                 f"{n_small} entries in {dt:.2f} s",
                                       "full_table_estimate_s": entries / (n_small / dt)}
-    print(json.dumps(result))
+    return result
 
 
 def bench_bresenham(args):
@@ -708,7 +708,7 @@ def bench_bresenham(args):
                                   "kind": "reference",
                                   "sample": f"BresenhamsLine numpy_calc_range over all {NQ} queries, {reps} passes "
                                             f"in {dt:.3f} s"}
-    print(json.dumps(result))
+    return result
 
 
 def bench_cddt(args):
@@ -784,7 +784,7 @@ def bench_cddt(args):
             result["cpu_baseline"] = {"value": (t2 - t0) * 1e3, "unit": "ms", "cores": 1, "kind": "reference",
                                       "sample": f"CDDTCast constructor ({t1 - t0:.2f} s) + prune ({t2 - t1:.2f} s) "
                                                 f"on a 512x512 corridor map, {TD} bins (1/16 of the cells)"}
-    print(json.dumps(result))
+    return result
 
 
 def main():
@@ -793,6 +793,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default workload: skip the other configurations' figures after the headline")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
@@ -813,18 +815,11 @@ def main():
     import range_libc_amd as rl
 
     dev = torch.device("cuda", 0)
-    if args.workload == "c1":
-        return bench_c1(args)
-    if args.workload == "uneven64":
-        return bench_uneven64(args)
-    if args.workload == "surrogate":
-        return bench_surrogate(args)
-    if args.workload == "glt":
-        return bench_glt(args)
-    if args.workload == "cddt":
-        return bench_cddt(args)
-    if args.workload == "bresenham":
-        return bench_bresenham(args)
+    legs = {"c1": bench_c1, "uneven64": bench_uneven64, "surrogate": bench_surrogate, "glt": bench_glt,
+            "cddt": bench_cddt, "bresenham": bench_bresenham}
+    if args.workload in legs:
+        print(json.dumps(legs[args.workload](args)))
+        return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.rand(n, device=dev)
@@ -1032,7 +1027,37 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(img, ins, angles, obs, args.cpu_budget)
+    if not args.no_secondary:
+        result["secondary"] = secondary_workloads(args, legs)
     print(json.dumps(result))
+
+
+def secondary_workloads(args, legs) -> dict:
+    """The other configurations, measured in the same run after the headline
+    (device time only, fewer timed launches, no CPU legs): the key figures of
+    each leg's own JSON line (`bench.py --workload NAME` prints it in full)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.no_cpu = max(20, args.steps // 10), 5, True
+    keys = ("metric", "value", "unit", "ms_per_step", "bl_ms", "rm_ms", "rays", "mismatches_vs_oracle",
+            "bl_sample_mismatches", "rm_sample_mismatches", "build_ms", "prune_ms_wall", "pf_step_glt_ms",
+            "pf_step_pcddt_ms", "pf_step_rm_ms", "surrogate_ms_per_step", "plain_ms_per_step", "train_s",
+            "autotuned")
+    out = {}
+    for name in ("c1", "bresenham", "uneven64", "glt", "cddt", "surrogate"):
+        t0 = time.perf_counter()
+        try:
+            r = legs[name](sub)
+            d = {k: r[k] for k in keys if k in r}
+            if "roofline" in r:
+                d["roofline_frac"] = r["roofline"].get("frac")
+            if name == "surrogate":
+                d["from_start_at_0.5"] = r["by_bound"][0]["from_start"]
+                d["err_p99_at_0.5"] = r["by_bound"][0]["err_p99"]
+            out[name] = d
+        except Exception as e:  # a failing leg is reported, not fatal to the headline line
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        out[name]["leg_wall_s"] = round(time.perf_counter() - t0, 2)
+    return out
 
 
 if __name__ == "__main__":
