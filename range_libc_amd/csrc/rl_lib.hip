@@ -300,12 +300,24 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
                                               int n_rays, int num_angles) {
   __shared__ float lut[RL_LUT_SIZE];
   stage_lut<METHOD>(m, lut);
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rays) return;
-  float x, y, h;
-  float scale = ray_pose<FRAME>(m, ins, angles, r, num_angles, &x, &y, &h);
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = r < n_rays;  // the whole wave stays to the end: rl::solo_tail needs every lane
+  float x = 0.0f, y = 0.0f, h = 0.0f, scale = 1.0f;
+  if (act) scale = ray_pose<FRAME>(m, ins, angles, r, num_angles, &x, &y, &h);
+  // ray marching over the compact map: a wave's last marching ray is finished
+  // by the whole wave (uniform across the wave)
+  const bool solo = RL_SOLO_TAIL && !COUNT && METHOD == RL_METHOD_RM && m.idx8;
+  rl::MarchTail tail{false, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u};
   int cnt = 0;
-  float d = cast_one<METHOD, COUNT>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, x, y, h, &cnt);
+  float d = 0.0f;
+  if (act) {
+    d = (solo && rl::fast_march_ok(x, y, max_range))
+            ? rl::rm_cast_idx8_t<false, false, true>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, x, y, h, &cnt,
+                                                     &tail)
+            : cast_one<METHOD, COUNT>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, x, y, h, &cnt);
+  }
+  if (solo) d = rl::solo_tail(m, lut, max_range, tail, d);
+  if (!act) return;
   if (COUNT)
     counts[r] = cnt;
   else
