@@ -430,31 +430,35 @@ __device__ __forceinline__ float solo_tail(const MapDev& m, const float* lut, fl
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
   const int k = (int)(threadIdx.x & (RL_SOLO_W - 1));  // lanes beyond RL_SOLO_W repeat the first ones
   int end, hx, hy;
+  const unsigned long long lanes = (1ull << RL_SOLO_W) - 1ull;
   for (;;) {
     float tk = t;
 #pragma unroll
     for (int j = 0; j < RL_SOLO_W - 1; ++j)
       if (j < k) tk = tk + g;
     const int px = (int)(x0 + dx * tk), py = (int)(y0 + dy * tk);
-    const bool live = tk < max_range;
     const bool inb = (unsigned)px < W && (unsigned)py < H;
+    // positions past max_range or off the map end the ray (decided before the load)
+    const unsigned long long out = __builtin_amdgcn_ballot_w64(!(tk < max_range) || !inb) & lanes;
     const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, inb ? (unsigned)px : 0u, inb ? (unsigned)py : 0u);
-    const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
-    const unsigned long long stop =
-        __builtin_amdgcn_ballot_w64(e != 0 || c != gcode) & ((1ull << RL_SOLO_W) - 1ull);
+    // a cell that is not g's code (a hit's code 0 never is) or an ending position stops the run
+    const unsigned long long stop = (__builtin_amdgcn_ballot_w64(c != gcode) & lanes) | out;
     const int j = stop ? __builtin_ctzll(stop) : RL_SOLO_W - 1;
     const float tj = rdf(tk, j);
     if (!stop) {  // every cell repeated g's code: RL_SOLO_W positions continued with step g
       t = tj + g;
       continue;
     }
-    const int ej = __builtin_amdgcn_readlane(e, j);
-    const int pxj = __builtin_amdgcn_readlane(px, j), pyj = __builtin_amdgcn_readlane(py, j);
-    if (ej) {
-      end = ej, hx = pxj, hy = pyj;
+    if ((out >> j) & 1ull) {  // past max_range or off the map: the range is max_range
+      end = 1, hx = 0, hy = 0;
       break;
     }
     const unsigned cj = (unsigned)__builtin_amdgcn_readlane((int)c, j);
+    const int pxj = __builtin_amdgcn_readlane(px, j), pyj = __builtin_amdgcn_readlane(py, j);
+    if (cj == 0u) {
+      end = 2, hx = pxj, hy = pyj;
+      break;
+    }
     g = idx8_step(m, lut, cj, pxj, pyj);
     gcode = (cj != RL_IDX8_ESCAPE) ? cj : 256u;
     t = tj + g;
