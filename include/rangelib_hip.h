@@ -334,6 +334,13 @@ int rl_debug_bf16_check(rl_ctx* ctx, unsigned long long* mismatches);
 /* C = A (M x K) * Bt (N x K)^T, the surrogate's forward-layer layout (the
  * 128 x 128 tile kernel when K % 32 == 0 and N % 128 == 0). */
 int rl_debug_gemm_bf16_nt(rl_ctx* ctx, const uint16_t* A, const uint16_t* Bt, float* C, int M, int N, int K);
+/* The CDDT build's device-wide exclusive sum (d_out[i] = d_in[0] + ... +
+ * d_in[i-1], i < n) and its per-segment ascending float sort (segment s is
+ * [d_start[s], d_start[s+1]), nseg + 1 offsets; -0 before +0), on device
+ * buffers.  Both return after the work ran. */
+int rl_debug_exclusive_sum_dev(rl_ctx* ctx, const unsigned long long* d_in, unsigned long long* d_out, long long n);
+int rl_debug_segmented_sort_dev(rl_ctx* ctx, const float* d_in, float* d_out, const long long* d_start,
+                                long long nseg);
 
 #ifdef __cplusplus
 }

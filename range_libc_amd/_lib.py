@@ -87,6 +87,8 @@ SIGNATURES = {
     "rl_debug_sincosf_dev": [_vp, _vp, _vp, _vp, C.c_int],
     "rl_debug_sqrtf_dev": [_vp, _vp, _vp, C.c_int],
     "rl_debug_chase_dev": [_vp, _vp, C.c_int, _vp, C.POINTER(C.c_double)],
+    "rl_debug_exclusive_sum_dev": [_vp, _vp, _vp, C.c_longlong],
+    "rl_debug_segmented_sort_dev": [_vp, _vp, _vp, _vp, C.c_longlong],
     "rl_debug_gemm_bf16": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "rl_debug_gemm_bf16_nt": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int],
     "rl_surrogate_create": [_vp, C.c_uint, C.POINTER(_vp)],
@@ -196,6 +198,14 @@ class Context:
         tick = C.c_double()
         check(lib().rl_debug_chase_dev(self.handle, _vp(next_ptr), steps, _vp(out_ptr), C.byref(tick)))
         return tick.value
+
+    def exclusive_sum_dev(self, in_ptr: int, out_ptr: int, n: int):
+        """The CDDT build's device-wide exclusive sum of uint64 counts."""
+        check(lib().rl_debug_exclusive_sum_dev(self.handle, _vp(in_ptr), _vp(out_ptr), n))
+
+    def segmented_sort_dev(self, in_ptr: int, out_ptr: int, start_ptr: int, nseg: int):
+        """The CDDT build's per-segment ascending float sort (int64 offsets)."""
+        check(lib().rl_debug_segmented_sort_dev(self.handle, _vp(in_ptr), _vp(out_ptr), _vp(start_ptr), nseg))
 
 
 _DEFAULT: dict[int, Context] = {}

@@ -8,13 +8,12 @@
 //   3. every (edge cell, angle < td/2) pair projects the cell centre into the
 //      bins its footprint covers: a counting pass (atomics), an exclusive scan
 //      into bin offsets, and a filling pass;
-//   4. each bin sorted (segmented radix sort) and its duplicates dropped;
+//   4. each bin sorted (segmented bitonic sort, rl_scan_sort.h) and its
+//      duplicates dropped;
 //   5. prune: every free cell marks the entries a query from it can return,
 //      unmarked entries are dropped.
 // The result equals the reference's compressed_lut bin for bin: the order in
 // which entries arrive does not matter after sorting and deduplication.
-
-#include <hipcub/hipcub.hpp>
 
 #define RL_CDDT_EPS 0.00001
 
@@ -242,6 +241,8 @@ struct DevBuf {
   }
 };
 
+#include "rl_scan_sort.h"
+
 void cddt_free(rl_map* m) {
   void* ps[] = {m->cddt_width, m->cddt_trans, m->cddt_cos, m->cddt_sin, m->cddt_bin_off, m->cddt_start, m->cddt_vals};
   for (void* p : ps)
@@ -256,13 +257,7 @@ void cddt_free(rl_map* m) {
 // Exclusive scan of counts[0..n) into out[0..n] (out[n] = total), on the stream.
 int scan_counts(hipStream_t st, const unsigned long long* counts, long long* out, long long n) {
   // the scan writes n + 1 values: counts is read with a trailing zero
-  size_t bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts, (unsigned long long*)out, (int)(n + 1), st));
-  DevBuf tmp;
-  TRY(tmp.alloc(bytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, counts, (unsigned long long*)out, (int)(n + 1), st));
-  HIP_TRY(hipStreamSynchronize(st));
-  return RL_OK;
+  return exclusive_sum_u64(st, counts, (unsigned long long*)out, n + 1);
 }
 
 // Keeps the entries selected by `mode` (see k_cddt_compact): new start and values.
@@ -369,16 +364,7 @@ int cddt_build_locked(rl_map* m, int td) {
   }
   // sort each bin
   HIP_TRY(hipMalloc(&m->cddt_vals, (size_t)(total ? total : 1) * sizeof(float)));
-  if (total > 0) {
-    size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, raw.as<float>(), m->cddt_vals, (int)total,
-                                                       (int)nb, m->cddt_start, m->cddt_start + 1, 0, 32, st));
-    DevBuf tmp;
-    TRY(tmp.alloc(bytes));
-    HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(tmp.p, bytes, raw.as<float>(), m->cddt_vals, (int)total,
-                                                       (int)nb, m->cddt_start, m->cddt_start + 1, 0, 32, st));
-    HIP_TRY(hipStreamSynchronize(st));
-  }
+  if (total > 0) TRY(segmented_sort_f32(st, raw.as<float>(), m->cddt_vals, m->cddt_start, nb));
   m->cddt_nv = total;
   // drop duplicates
   return cddt_compact(m, 0, nullptr);
@@ -521,6 +507,20 @@ int rl_map_get_cddt(rl_map* m, int* theta_disc, long long* nbins, long long* nva
   if (vals && m->cddt_nv)
     HIP_TRY(hipMemcpy(vals, m->cddt_vals, (size_t)m->cddt_nv * sizeof(float), hipMemcpyDeviceToHost));
   return RL_OK;
+}
+
+int rl_debug_exclusive_sum_dev(rl_ctx* c, const unsigned long long* in, unsigned long long* out, long long n) {
+  CHECK_ARG(c && (n <= 0 || (in && out)), "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  return exclusive_sum_u64(c->stream, in, out, n);
+}
+
+int rl_debug_segmented_sort_dev(rl_ctx* c, const float* in, float* out, const long long* start, long long nseg) {
+  CHECK_ARG(c && (nseg <= 0 || (in && out && start)), "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  TRY(use_device(c));
+  return segmented_sort_f32(c->stream, in, out, start, nseg);
 }
 
 }  // extern "C"
