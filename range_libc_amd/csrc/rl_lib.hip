@@ -864,6 +864,8 @@ int segment_size(int m) {
 
 }  // namespace
 
+#define RL_PINNED_BYTES (512u << 10)  // host-pointer calls whose parts fit run zero-copy (rl_ctx::pinned)
+
 struct rl_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -880,6 +882,22 @@ struct rl_ctx {
   void* d_scratch = nullptr;  // this device's copy of `scratch` (no-allocation entry point)
   void* stage[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   size_t stage_cap[6] = {0, 0, 0, 0, 0, 0};
+  void* pin = nullptr;  // RL_PINNED_BYTES of pinned host memory for small host-pointer calls
+
+  // Small host-pointer calls copy their inputs into this pinned block on the
+  // host and the kernel reads and writes it in place (it is mapped into the
+  // device's address space): one launch and one synchronisation per call
+  // instead of two or more copies through device staging.  Calls hold the
+  // context's mutex until they have synchronised, so one block serves all.
+  int pinned(void** out) {
+    if (!pin && hipHostMalloc(&pin, RL_PINNED_BYTES, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      pin = nullptr;
+      return fail(RL_ERR_NOMEM, "hipHostMalloc of %u pinned bytes failed", (unsigned)RL_PINNED_BYTES);
+    }
+    *out = pin;
+    return RL_OK;
+  }
 
   // Device staging buffer `slot` of at least `bytes` (host-pointer entry points).
   int staging(int slot, size_t bytes, void** out) {
@@ -1365,6 +1383,7 @@ int rl_destroy(rl_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 6; ++i)
     if (c->stage[i]) (void)hipFree(c->stage[i]);
+  if (c->pin) (void)hipHostFree(c->pin);
   if (c->trig_buf) (void)hipFree(c->trig_buf);
   if (c->d_lut) (void)hipFree(c->d_lut);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1619,6 +1638,8 @@ int rl_map_edt_build_ms(rl_map* m, float* ms) {
 
 // --- host-pointer batches ----------------------------------------------------
 
+static size_t pin_part(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
 static int host_cast(rl_map* m, int method, int frame, float max_range, const float* ins,
                      const float* angles, float* outs, int num_poses, int num_angles) {
   CHECK_ARG(m && ins && outs, "null argument");
@@ -1634,6 +1655,20 @@ static int host_cast(rl_map* m, int method, int frame, float max_range, const fl
   long long n_rays = frame == RL_FRAME_ANGLES ? (long long)num_poses * num_angles : num_poses;
   if (n_rays > 0x7fffffffLL) return fail(RL_ERR_ARG, "batch of %lld rays exceeds 2^31-1", n_rays);
   if (n_rays == 0) return RL_OK;
+  const size_t b_in = pin_part((size_t)num_poses * 3 * sizeof(float));
+  const size_t b_ang = frame == RL_FRAME_ANGLES ? pin_part((size_t)num_angles * sizeof(float)) : 0;
+  const size_t b_out = pin_part((size_t)n_rays * sizeof(float));
+  if (b_in + b_ang + b_out <= RL_PINNED_BYTES) {
+    char* p;
+    TRY(c->pinned((void**)&p));
+    memcpy(p, ins, (size_t)num_poses * 3 * sizeof(float));
+    if (b_ang) memcpy(p + b_in, angles, (size_t)num_angles * sizeof(float));
+    TRY(launch_cast(m, method, frame, false, max_range, (const float*)p, b_ang ? (const float*)(p + b_in) : nullptr,
+                    (float*)(p + b_in + b_ang), nullptr, (int)n_rays, num_angles));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(outs, p + b_in + b_ang, (size_t)n_rays * sizeof(float));
+    return RL_OK;
+  }
   void *d_ins, *d_ang = nullptr, *d_out;
   TRY(c->staging(0, (size_t)num_poses * 3 * sizeof(float), &d_ins));
   TRY(c->staging(1, (size_t)n_rays * sizeof(float), &d_out));
@@ -1678,6 +1713,17 @@ int rl_eval_sensor_model(rl_map* m, const float* obs, const float* ranges, doubl
   std::lock_guard<std::mutex> lk(c->mu);
   TRY(use_device(c));
   if (N == 0) return RL_OK;
+  const size_t b_obs = pin_part((size_t)M * sizeof(float)), b_rng = pin_part((size_t)M * N * sizeof(float));
+  if (b_obs + b_rng + pin_part((size_t)N * sizeof(double)) <= RL_PINNED_BYTES) {
+    char* p;
+    TRY(c->pinned((void**)&p));
+    memcpy(p, obs, (size_t)M * sizeof(float));
+    memcpy(p + b_obs, ranges, (size_t)M * N * sizeof(float));
+    TRY(launch_sensor(m, (const float*)p, (const float*)(p + b_obs), (double*)(p + b_obs + b_rng), M, N));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(outs, p + b_obs + b_rng, (size_t)N * sizeof(double));
+    return RL_OK;
+  }
   void *d_obs, *d_rng, *d_w;
   TRY(c->staging(2, (size_t)M * sizeof(float), &d_obs));
   TRY(c->staging(3, (size_t)M * N * sizeof(float), &d_rng));
@@ -1699,6 +1745,19 @@ int rl_calc_range_repeat_angles_eval_sensor_model(rl_map* m, int method, float m
   std::lock_guard<std::mutex> lk(c->mu);
   TRY(use_device(c));
   if (N == 0) return RL_OK;
+  const size_t b_in = pin_part((size_t)N * 3 * sizeof(float)), b_ang = pin_part((size_t)M * sizeof(float));
+  if (b_in + 2 * b_ang + pin_part((size_t)N * sizeof(double)) <= RL_PINNED_BYTES) {
+    char* p;
+    TRY(c->pinned((void**)&p));
+    memcpy(p, ins, (size_t)N * 3 * sizeof(float));
+    memcpy(p + b_in, angles, (size_t)M * sizeof(float));
+    memcpy(p + b_in + b_ang, obs, (size_t)M * sizeof(float));
+    TRY(launch_fused(m, method, max_range, (const float*)p, (const float*)(p + b_in),
+                     (const float*)(p + b_in + b_ang), (double*)(p + b_in + 2 * b_ang), N, M));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(weights, p + b_in + 2 * b_ang, (size_t)N * sizeof(double));
+    return RL_OK;
+  }
   void *d_ins, *d_ang, *d_obs, *d_w;
   TRY(c->staging(0, (size_t)N * 3 * sizeof(float), &d_ins));
   TRY(c->staging(2, (size_t)M * sizeof(float), &d_ang));

@@ -215,6 +215,33 @@ def test_fused_sensor_model_bit_exact(kind, m, corridor_map):
     assert np.array_equal(w, orc.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))
 
 
+@pytest.mark.parametrize("n,m", [(50, 60), (3000, 60), (50_000, 8)])
+def test_host_calls_zero_copy_and_staged(n, m, corridor_map):
+    """The host-pointer sensor-model and fused calls run zero-copy through a
+    pinned block when their arrays fit 512 KiB (50 x 60) and through device
+    staging otherwise (3000 x 60 ranges: 720 KB; 50,000 poses: 600 KB);
+    both give the oracle's weights."""
+    table = W.sensor_table(501)
+    R = 300.0
+    ins = W.world_poses(corridor_map, n, seed=40 + m)
+    ang = W.beam_angles(m)
+    orc = O.Oracle(corridor_map, R, "rm")
+    orc.set_sensor_model(table)
+    obs = W.noisy_observation(orc.numpy_calc_range_angles(W.world_poses(corridor_map, 1, seed=41), ang), seed=42,
+                              max_range=R)
+    meth = rl.PyRayMarchingGPU(rl.PyOMap(corridor_map), R)
+    meth.set_sensor_model(table)
+    ranges = np.empty(n * m, np.float32)
+    meth.calc_range_repeat_angles(ins, ang, ranges)
+    assert_bits_equal(ranges, orc.numpy_calc_range_angles(ins, ang), "ranges")
+    w = np.empty(n, np.float64)
+    meth.eval_sensor_model(obs, ranges, w, m, n)
+    assert np.array_equal(w, orc.eval_sensor_model(obs, ranges, m, n))
+    wf = np.empty(n, np.float64)
+    meth.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs, wf)
+    assert np.array_equal(wf, orc.calc_range_repeat_angles_eval_sensor_model(ins, ang, obs))
+
+
 def test_sensor_model_requires_table():
     meth = rl.PyRayMarchingGPU(rl.PyOMap(np.zeros((8, 8), np.uint8)), 10.0)
     with pytest.raises(rl.RangeLibError):
