@@ -368,7 +368,12 @@ def bench_uneven64(args):
     setup_s = time.perf_counter() - t0
     batch = rl.Batch(ctx)
     batch.set_jobs(jobs)
-    tuned_ms = batch.autotune(reps=3)
+    if args.batch_config:  # a fixed configuration (counter passes over one configuration)
+        rpw, box, order, grid = (int(v) for v in args.batch_config.split(","))
+        batch.set_config(rays_per_wave=rpw, lds_box=box, order=order, grid=grid)
+        tuned_ms = None
+    else:
+        tuned_ms = batch.autotune(reps=3)
 
     def timed(fn, reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -417,6 +422,19 @@ def bench_uneven64(args):
                      "kernel_ms": batch_ms, "alg_bytes_per_launch": alg, "edt_reads_per_ray": reads / total_rays,
                      "max_reads_per_ray": max_reads},
         "config": {"workload": "uneven64_side256-4096_p500-8000_b30-120_r500"}}
+    # HBM traffic and instruction issue of the launch, from the committed counter passes
+    tpath = os.path.join(ROOT, "profiles", "traffic_batch_uneven64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        result["roofline"]["traffic"] = tj["hbm_bytes_per_launch"]
+        result["roofline"]["traffic_source"] = (
+            f"profiles/traffic_batch_uneven64.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this "
+            f"launch in configuration {tj['config']} (profiled launches {tj['profiled_launch_us']} us), scaled by "
+            f"a 256 MiB copy's counts; not measured in this run")
+        result["issue_bound"] = {"valu_us": tj["issue"]["valu_issue_bound_us"],
+                                 "salu_us": tj["issue"]["salu_issue_bound_us"],
+                                 "wait_frac": tj["issue"]["wait_frac"], "source": "the same file's SQ passes"}
     if not args.no_cpu:
         result["cpu_baseline"] = uneven64_cpu_baseline(specs, table, keep, args.cpu_budget)
     return result
@@ -798,6 +816,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--xcd", type=int, default=0, help="XCD-aware particle scheduling (1) or input order (0)")
     ap.add_argument("--box", type=int, default=0, help="stage the map around each particle in LDS (1) or not (0)")
+    ap.add_argument("--batch-config", default="", metavar="RPW,BOX,ORDER,GRID",
+                    help="uneven64: this batch configuration instead of the autotuner's")
     ap.add_argument("--workload", default="pf_step",
                     choices=["pf_step", "c1", "uneven64", "surrogate", "glt", "cddt", "bresenham"],
                     help="pf_step (the headline particle-filter step), uneven64 (multi-map batch) or "

@@ -259,6 +259,9 @@ int rl_batch_set_jobs(rl_batch* batch, const rl_batch_job* jobs, int n_jobs);
  * work order, grid size) `reps` times on the current jobs and keeps the
  * fastest; *best_ms receives its time per launch. */
 int rl_batch_autotune(rl_batch* batch, int reps, float* best_ms);
+/* order: 0 job order, 1 estimated cost descending, 2 each map's work on one
+ * XCD (balanced by estimated cost, longest first); grid: 0 one workgroup per
+ * work item, else a persistent grid of that many workgroups. */
 int rl_batch_set_config(rl_batch* batch, int rays_per_wave, int lds_box, int order, int grid);
 int rl_batch_get_config(rl_batch* batch, int* rays_per_wave, int* lds_box, int* order, int* grid);
 /* One launch over all jobs on the context's stream: no allocation, no copy,

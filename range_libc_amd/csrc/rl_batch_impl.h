@@ -10,7 +10,12 @@
 //
 // Tuned per batch by rl_batch_autotune: lanes per particle S (16, 32 or 64,
 // never above the beam count's power of two), LDS staging of the 64x64 cells
-// around each particle (S = 64 only), item order, and grid size.  The
+// around each particle (S = 64 only), item order, and grid size.  Order 2
+// keeps each map on one XCD: workgroup b runs on XCD b mod X (X XCDs, each
+// with its own L2), so the list is X interleaved queues, each map's items in
+// one queue (a map costing more than a queue's share spills into the next
+// least loaded one), queues balanced by estimated cost, longest items first
+// within a queue, and padded with empty items to equal length.  The
 // estimated cost of an item is its ray count times the mean number of
 // distance-map reads per ray of its map, measured on a sample of the map's
 // own rays when the jobs are set.
@@ -30,7 +35,7 @@ struct BatchMap {
 };
 
 struct BatchItem {
-  int map;
+  int map;    // -1: an empty item (padding of the XCD-affine order)
   int first;  // first particle of the chunk
 };
 
@@ -49,6 +54,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
   const int wib = threadIdx.x >> 6;
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const BatchItem item = items[it];
+    if (item.map < 0) continue;
     const BatchMap& bm = maps[item.map];
     MapDev m = bm.m;
     m.use_box = use_box;
@@ -78,8 +84,9 @@ struct rl_batch {
   // launch configuration
   int rays_per_wave = 64;  // upper bound on S
   int lds_box = 0;
-  int order = 1;           // 0: job order, 1: estimated cost descending
+  int order = 1;           // 0: job order, 1: estimated cost descending, 2: XCD-affine (see above)
   int grid = 0;            // 0: one workgroup per item
+  int xcds = 8;            // XCDs of the device (workgroup b runs on XCD b mod xcds)
   // device descriptors of the current configuration
   BatchMap* d_maps = nullptr;
   BatchItem* d_items = nullptr;
@@ -121,13 +128,48 @@ int batch_upload(rl_batch* b) {
       items.push_back({cnt * ray_cost, BatchItem{i, f}});
     }
   }
-  if (b->order == 1)
-    std::stable_sort(items.begin(), items.end(),
-                     [](const std::pair<double, BatchItem>& a, const std::pair<double, BatchItem>& z) {
-                       return a.first > z.first;
-                     });
-  std::vector<BatchItem> flat(items.size());
-  for (size_t k = 0; k < items.size(); ++k) flat[k] = items[k].second;
+  auto by_cost = [](const std::pair<double, BatchItem>& a, const std::pair<double, BatchItem>& z) {
+    return a.first > z.first;
+  };
+  std::vector<BatchItem> flat;
+  if (b->order == 2 && b->xcds > 1) {
+    const int X = b->xcds;
+    std::vector<double> jcost(nj, 0.0);
+    std::vector<std::vector<std::pair<double, BatchItem>>> per_job(nj);
+    double total = 0.0;
+    for (const auto& it : items) {
+      jcost[it.second.map] += it.first;
+      per_job[it.second.map].push_back(it);
+      total += it.first;
+    }
+    std::vector<int> jorder(nj);
+    for (int i = 0; i < nj; ++i) jorder[i] = i;
+    std::stable_sort(jorder.begin(), jorder.end(), [&](int a, int z) { return jcost[a] > jcost[z]; });
+    const double share = total / X;
+    std::vector<double> load(X, 0.0);
+    std::vector<std::vector<std::pair<double, BatchItem>>> q(X);
+    auto least = [&]() { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
+    for (int j : jorder) {
+      int x = least();
+      for (const auto& it : per_job[j]) {
+        if (load[x] >= share) x = least();  // this map spills into the next queue
+        q[x].push_back(it);
+        load[x] += it.first;
+      }
+    }
+    size_t len = 0;
+    for (auto& v : q) {
+      std::stable_sort(v.begin(), v.end(), by_cost);
+      len = std::max(len, v.size());
+    }
+    flat.assign(len * X, BatchItem{-1, 0});
+    for (int x = 0; x < X; ++x)
+      for (size_t k = 0; k < q[x].size(); ++k) flat[k * X + x] = q[x][k].second;
+  } else {
+    if (b->order == 1) std::stable_sort(items.begin(), items.end(), by_cost);
+    flat.resize(items.size());
+    for (size_t k = 0; k < items.size(); ++k) flat[k] = items[k].second;
+  }
   if (b->cap_maps < maps.size()) {
     if (b->d_maps) HIP_TRY(hipFree(b->d_maps));
     b->d_maps = nullptr;
@@ -184,6 +226,10 @@ int batch_launch_locked(rl_batch* b) {
   if (b->n_items == 0) return RL_OK;
   rl_ctx* c = b->ctx;
   int grid = b->grid > 0 && b->grid < b->n_items ? b->grid : b->n_items;
+  if (b->order == 2 && grid < b->n_items) {  // keep item i on workgroups of XCD i mod xcds
+    grid -= grid % b->xcds;
+    if (grid == 0) grid = b->n_items;
+  }
   if (b->method == RL_METHOD_BL)
     hipLaunchKernelGGL(k_batch<RL_METHOD_BL>, dim3(grid), dim3(256), 0, c->stream, b->d_maps, b->d_items,
                        b->n_items, c->trig, b->lds_box);
@@ -229,6 +275,11 @@ int rl_batch_create(rl_ctx* c, int method, rl_batch** out) {
   TRY(check_method(method));
   rl_batch* b = new rl_batch;
   b->ctx = c;
+  {
+    int x = 0;
+    if (hipDeviceGetAttribute(&x, hipDeviceAttributeNumberOfXccs, c->device) == hipSuccess && x > 0) b->xcds = x;
+    (void)hipGetLastError();
+  }
   c->refs.fetch_add(1);
   b->method = method;
   *out = b;
@@ -278,7 +329,7 @@ int rl_batch_set_jobs(rl_batch* b, const rl_batch_job* jobs, int n_jobs) {
 int rl_batch_set_config(rl_batch* b, int rays_per_wave, int lds_box, int order, int grid) {
   CHECK_ARG(b, "batch is NULL");
   CHECK_ARG(rays_per_wave == 16 || rays_per_wave == 32 || rays_per_wave == 64, "rays_per_wave must be 16, 32 or 64");
-  CHECK_ARG(order == 0 || order == 1, "order must be 0 or 1");
+  CHECK_ARG(order >= 0 && order <= 2, "order must be 0, 1 or 2");
   CHECK_ARG(grid >= 0, "grid must be >= 0");
   std::lock_guard<std::mutex> lk(b->ctx->mu);
   TRY(use_device(b->ctx));
@@ -326,7 +377,7 @@ int rl_batch_autotune(rl_batch* b, int reps, float* best_ms) {
   for (int w : waves) {
     for (int box = 0; box < 2 && rc == RL_OK; ++box) {
       if (box && w != 64) continue;  // the LDS box needs one particle per wave
-      for (int order = 0; order < 2 && rc == RL_OK; ++order) {
+      for (int order = 0; order < 3 && rc == RL_OK; ++order) {
         for (int g : grids) {
           b->rays_per_wave = w;
           b->lds_box = box;

@@ -76,8 +76,8 @@ def test_batch_every_config_bit_exact():
     b.set_jobs(jobs)
     for rpw in (64, 32, 16):
         for box in ((0, 1) if rpw == 64 else (0,)):
-            for order in (0, 1):
-                for grid in (0, 5):
+            for order in (0, 1, 2):
+                for grid in (0, 5, 16):
                     for _, d in keep:
                         d["rng"].fill_(-1)
                         d["w"].fill_(-1)
