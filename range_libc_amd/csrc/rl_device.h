@@ -176,6 +176,27 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
 #define RL_SPEC_AFTER 8  // reads of a ray before it starts to speculate
 #endif
 
+// Compact-map reads go through a buffer resource covering the map: a read
+// past its end returns 0 instead of faulting, and a position off the map
+// (negative or too large coordinates wrap in the 32-bit offset) reads either
+// that 0 or some cell of the map.  Every march decides off-map positions from
+// the position alone, before it looks at the code, so those reads need no
+// clamp of the position (two selects per read less on the march's chain).
+#ifndef RL_IDX8_BUFFER
+#define RL_IDX8_BUFFER 1
+#endif
+#if RL_IDX8_BUFFER
+#define RL_CELL(inb, v) ((unsigned)(v))
+#else
+#define RL_CELL(inb, v) ((inb) ? (unsigned)(v) : 0u)
+#endif
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t idx8_rsrc(const MapDev& m) {
+  // the compact map is built only below 4 GiB; 0x00020000: 32-bit data format
+  const unsigned bytes = ((unsigned)((m.width + 7) >> 3) * (unsigned)m.nby) << 6;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)m.idx8, (short)0, (int)bytes, 0x00020000);
+}
+
 // Code of grid cell (ux, uy) of the compact map: from the LDS box when BOX and
 // the box covers the cell, else from the map in HBM.
 template <bool BOX>
@@ -188,7 +209,12 @@ __device__ __forceinline__ unsigned idx8_code(const MapDev& m, const Box& box, u
   // 32-bit offset from the map's base (the compact map is built only below
   // 4 GiB, so 64 * nby < 2^24): a multiply-add on the march's chain and a
   // scalar-base load
-  return glb(m.idx8)[__umul24(ux >> 3, (unsigned)m.nby << 6) + ((uy << 3) | (ux & 7u))];
+  const unsigned off = __umul24(ux >> 3, (unsigned)m.nby << 6) + ((uy << 3) | (ux & 7u));
+#if RL_IDX8_BUFFER
+  return __builtin_amdgcn_raw_buffer_load_b8(idx8_rsrc(m), off, 0, 0);
+#else
+  return glb(m.idx8)[off];
+#endif
 }
 
 // March step max(d * 0.999f, 1.0f) from a cell with non-zero code c: from the
@@ -246,7 +272,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
     const bool live = t < max_range;
     const bool inb = (unsigned)px < W && (unsigned)py < H;
-    const unsigned c = idx8_code<BOX>(m, box, inb ? (unsigned)px : 0u, inb ? (unsigned)py : 0u);
+    const unsigned c = idx8_code<BOX>(m, box, RL_CELL(inb, px), RL_CELL(inb, py));
     const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
     if (COUNT) nr += (live && inb) ? 1 : 0;
     if (e) {
@@ -268,8 +294,8 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
     const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
     const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
-    const unsigned c0 = idx8_code<BOX>(m, box, in0 ? (unsigned)px0 : 0u, in0 ? (unsigned)py0 : 0u);
-    const unsigned c1 = idx8_code<BOX>(m, box, in1 ? (unsigned)px1 : 0u, in1 ? (unsigned)py1 : 0u);
+    const unsigned c0 = idx8_code<BOX>(m, box, RL_CELL(in0, px0), RL_CELL(in0, py0));
+    const unsigned c1 = idx8_code<BOX>(m, box, RL_CELL(in1, px1), RL_CELL(in1, py1));
     const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
     const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
     const bool use1 = e0 == 0 && c0 == gcode;  // position 1 is the true next position
@@ -295,9 +321,9 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
     const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
     const bool in2 = (unsigned)px2 < W && (unsigned)py2 < H;
-    const unsigned c0 = idx8_code<BOX>(m, box, in0 ? (unsigned)px0 : 0u, in0 ? (unsigned)py0 : 0u);
-    const unsigned c1 = idx8_code<BOX>(m, box, in1 ? (unsigned)px1 : 0u, in1 ? (unsigned)py1 : 0u);
-    const unsigned c2 = idx8_code<BOX>(m, box, in2 ? (unsigned)px2 : 0u, in2 ? (unsigned)py2 : 0u);
+    const unsigned c0 = idx8_code<BOX>(m, box, RL_CELL(in0, px0), RL_CELL(in0, py0));
+    const unsigned c1 = idx8_code<BOX>(m, box, RL_CELL(in1, px1), RL_CELL(in1, py1));
+    const unsigned c2 = idx8_code<BOX>(m, box, RL_CELL(in2, px2), RL_CELL(in2, py2));
     const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
     const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
     const int e2 = (!(t2 < max_range) || !in2) ? 1 : (c2 == 0u ? 2 : 0);
@@ -329,7 +355,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     }
 #pragma unroll
     for (int k = 0; k < RL_SPEC; ++k)
-      c[k] = idx8_code<BOX>(m, box, in[k] ? (unsigned)px[k] : 0u, in[k] ? (unsigned)py[k] : 0u);
+      c[k] = idx8_code<BOX>(m, box, RL_CELL(in[k], px[k]), RL_CELL(in[k], py[k]));
 #pragma unroll
     for (int k = 0; k < RL_SPEC; ++k) e[k] = (!(tk[k] < max_range) || !in[k]) ? 1 : (c[k] == 0u ? 2 : 0);
     int sel = RL_SPEC - 1;
@@ -385,7 +411,12 @@ __device__ __forceinline__ float rm_cast_idx8_t(const MapDev& m, const float* lu
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
   const int px = (int)x0, py = (int)y0;
   const bool in = (unsigned)px < W && (unsigned)py < H;
+#if RL_IDX8_BUFFER
+  const unsigned c_any = idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py);
+  const unsigned c = in ? c_any : 0u;
+#else
   const unsigned c = in ? idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py) : 0u;
+#endif
   float dy, dx;
   sincos_libc(heading, trig, &dy, &dx);
   if (COUNT) *reads = 0;
@@ -440,7 +471,7 @@ __device__ __forceinline__ float solo_tail(const MapDev& m, const float* lut, fl
     const bool inb = (unsigned)px < W && (unsigned)py < H;
     // positions past max_range or off the map end the ray (decided before the load)
     const unsigned long long out = __builtin_amdgcn_ballot_w64(!(tk < max_range) || !inb) & lanes;
-    const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, inb ? (unsigned)px : 0u, inb ? (unsigned)py : 0u);
+    const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, RL_CELL(inb, px), RL_CELL(inb, py));
     // a cell that is not g's code (a hit's code 0 never is) or an ending position stops the run
     const unsigned long long stop = (__builtin_amdgcn_ballot_w64(c != gcode) & lanes) | out;
     const int j = stop ? __builtin_ctzll(stop) : RL_SOLO_W - 1;
