@@ -267,7 +267,9 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   // (t, g, gcode, nr): position, last step, the code that gave it (256 matches
   // no cell) and reads so far; (0, 1, 1, 0) for a march from its origin
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
-  int end = 0, hx = 0, hy = 0;
+  // the loops leave t at the position that ended the ray; a hit's cell is
+  // formed again from it after the loops (the same float operations)
+  int end = 0;
   for (int i = nr; RL_SPEC == 1 || i < RL_SPEC_AFTER; ++i) {
     const int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
     const bool live = t < max_range;
@@ -276,7 +278,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
     if (COUNT) nr += (live && inb) ? 1 : 0;
     if (e) {
-      end = e, hx = px, hy = py;
+      end = e;
       break;
     }
     g = idx8_step(m, lut, c, px, py);
@@ -305,13 +307,14 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     if (COUNT) nr += (use1 ? 1 : 0) + (es != 1 ? 1 : 0);
     const int sx = use1 ? px1 : px0, sy = use1 ? py1 : py0;
     const unsigned cs = use1 ? c1 : c0;
+    const float ts = use1 ? t1 : t0;
     if (es) {
-      end = es, hx = sx, hy = sy;
+      end = es, t = ts;
       break;
     }
     g = idx8_step(m, lut, cs, sx, sy);
     gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
-    t = (use1 ? t1 : t0) + g;
+    t = ts + g;
   }
   while (RL_SPEC == 3 && !end) {  // width 3, written out
     const float t0 = t, t1 = t + g, t2 = t1 + g;
@@ -333,13 +336,14 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     if (COUNT) nr += (u1 ? 1 : 0) + (u2 ? 1 : 0) + (es != 1 ? 1 : 0);
     const int sx = u2 ? px2 : (u1 ? px1 : px0), sy = u2 ? py2 : (u1 ? py1 : py0);
     const unsigned cs = u2 ? c2 : (u1 ? c1 : c0);
+    const float ts = u2 ? t2 : (u1 ? t1 : t0);
     if (es) {
-      end = es, hx = sx, hy = sy;
+      end = es, t = ts;
       break;
     }
     g = idx8_step(m, lut, cs, sx, sy);
     gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
-    t = (u2 ? t2 : (u1 ? t1 : t0)) + g;
+    t = ts + g;
   }
   while (RL_SPEC > 3 && !end) {  // any width (slower selects than the written-out forms)
     float tk[RL_SPEC];
@@ -370,7 +374,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
       if (sel == k) es = e[k], sx = px[k], sy = py[k], cs = c[k], ts = tk[k];
     if (COUNT) nr += sel + (es != 1 ? 1 : 0);
     if (es) {
-      end = es, hx = sx, hy = sy;
+      end = es, t = ts;
       break;
     }
     g = idx8_step(m, lut, cs, sx, sy);
@@ -379,6 +383,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   }
   if (COUNT) *reads = nr;
   if (end == 2) {
+    const int hx = (int)(x0 + dx * t), hy = (int)(y0 + dy * t);
     const float xd = (float)hx - x0;
     const float yd = (float)hy - y0;
     return __builtin_sqrtf(xd * xd + yd * yd);
