@@ -287,9 +287,11 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   }
   while (RL_SPEC == 2 && !end) {  // width 2, written out
     if (SOLO && __popcll(__builtin_amdgcn_read_exec()) == 1) {
-      // the wave's last marching lane: the whole wave takes over (solo_tail)
-      *tail = MarchTail{true, x0, y0, dx, dy, t, g, gcode};
-      return 0.0f;
+      // the wave's last marching lane: handed to the whole wave after the
+      // loops (one exit block for every way out of the loop: a return from
+      // here cost register copies of t, g and gcode in every round)
+      end = 3;
+      break;
     }
     const float t0 = t, t1 = t + g;
     const int px0 = (int)(x0 + dx * t0), py0 = (int)(y0 + dy * t0);
@@ -380,6 +382,10 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     g = idx8_step(m, lut, cs, sx, sy);
     gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
     t = ts + g;
+  }
+  if (SOLO && end == 3) {  // the whole wave takes over (solo_tail)
+    *tail = MarchTail{true, x0, y0, dx, dy, t, g, gcode};
+    return 0.0f;
   }
   if (COUNT) *reads = nr;
   if (end == 2) {
