@@ -487,22 +487,21 @@ __device__ __forceinline__ float solo_tail(const MapDev& m, const float* lut, fl
     const unsigned long long stop = (__builtin_amdgcn_ballot_w64(c != gcode) & lanes) | out;
     const int j = stop ? __builtin_ctzll(stop) : RL_SOLO_W - 1;
     const float tj = rdf(tk, j);
-    if (!stop) {  // every cell repeated g's code: RL_SOLO_W positions continued with step g
-      t = tj + g;
-      continue;
-    }
-    if ((out >> j) & 1ull) {  // past max_range or off the map: the range is max_range
-      end = 1, hx = 0, hy = 0;
-      break;
-    }
+    // one way out of the loop (several cost register copies in every round):
+    // the ray ends at position j when it is past max_range or off the map, or
+    // on a hit; otherwise a stop takes the step of j's cell, and no stop keeps
+    // g (every cell repeated g's code)
     const unsigned cj = (unsigned)__builtin_amdgcn_readlane((int)c, j);
     const int pxj = __builtin_amdgcn_readlane(px, j), pyj = __builtin_amdgcn_readlane(py, j);
-    if (cj == 0u) {
-      end = 2, hx = pxj, hy = pyj;
+    const bool is_out = (out >> j) & 1ull;
+    if (is_out || (stop && cj == 0u)) {
+      end = is_out ? 1 : 2, hx = pxj, hy = pyj;
       break;
     }
-    g = idx8_step(m, lut, cj, pxj, pyj);
-    gcode = (cj != RL_IDX8_ESCAPE) ? cj : 256u;
+    if (stop) {
+      g = idx8_step(m, lut, cj, pxj, pyj);
+      gcode = (cj != RL_IDX8_ESCAPE) ? cj : 256u;
+    }
     t = tj + g;
   }
   float r = max_range;
