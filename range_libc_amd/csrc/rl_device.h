@@ -799,28 +799,41 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
   // When steep the roles of (x, y) swap back for the map lookup.
   float limx = steep ? fw : fh;  // bound on _x
   float limy = steep ? fh : fw;  // bound on _y
-  for (int it = 0; trunc_x86(_x) != xend && it < cap; ++it) {
+  // One way out of the loop: the body has no branch (the cell is read through
+  // a buffer resource covering the map, so a step off the map reads 0 or an
+  // ignored cell), and the loop test is the reference's, on the updated _x.
+  const __amdgpu_buffer_rsrc_t occ_rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)occ, (short)0, (int)((unsigned)W * (unsigned)H), 0x00020000);
+  bool hit = false;
+  int it = 0;
+  bool go = trunc_x86(_x) != xend && it < cap;
+  while (go) {
     _x += xstep;
     error += deltay;
     if (COUNT) ++nv;
     // error * 2.00 >= deltax is evaluated in double (RangeLib.h:749); doubling
     // a float is exact, so the float comparison is the same test.
-    if (error * 2.0f >= deltax) {
-      _y += ystep;
-      error -= deltax;
-    }
-    if (0.0f <= _y && _y < limy && 0.0f <= _x && _x < limx) {
-      int cx = steep ? (int)_x : (int)_y;
-      int cy = steep ? (int)_y : (int)_x;
-      if (glb(occ)[__umul24((unsigned)cx, (unsigned)H) + (unsigned)cy]) {  // W, H <= 65535: 32-bit
-        float xd = _x - x0;
-        float yd = _y - y0;
-        if (COUNT) *visits = nv;
-        return __builtin_sqrtf(xd * xd + yd * yd);
-      }
-    }
+    const bool adv = error * 2.0f >= deltax;
+    _y = adv ? _y + ystep : _y;
+    error = adv ? error - deltax : error;
+    const bool inb = (0.0f <= _y) & (_y < limy) & (0.0f <= _x) & (_x < limx);  // no short-circuit branch
+    // _x stays below 2^23 + 2 in magnitude (checked above); _y, off the map,
+    // may not: it is clamped into int range before the conversion (the
+    // clamp leaves every position on the map unchanged)
+    const float ym = __builtin_amdgcn_fmed3f(_y, -1.0f, 1073741824.0f);
+    const int cx = steep ? (int)_x : (int)ym;
+    const int cy = steep ? (int)ym : (int)_x;
+    const unsigned o = __builtin_amdgcn_raw_buffer_load_b8(occ_rsrc, __umul24((unsigned)cx, (unsigned)H) + (unsigned)cy, 0, 0);
+    hit = inb & (o != 0u);
+    ++it;
+    go = !hit & (trunc_x86(_x) != xend) & (it < cap);
   }
   if (COUNT) *visits = nv;
+  if (hit) {
+    float xd = _x - x0;
+    float yd = _y - y0;
+    return __builtin_sqrtf(xd * xd + yd * yd);
+  }
   return max_range;
 }
 
