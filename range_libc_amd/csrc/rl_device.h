@@ -182,15 +182,6 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
 // that 0 or some cell of the map.  Every march decides off-map positions from
 // the position alone, before it looks at the code, so those reads need no
 // clamp of the position (two selects per read less on the march's chain).
-#ifndef RL_IDX8_BUFFER
-#define RL_IDX8_BUFFER 1
-#endif
-#if RL_IDX8_BUFFER
-#define RL_CELL(inb, v) ((unsigned)(v))
-#else
-#define RL_CELL(inb, v) ((inb) ? (unsigned)(v) : 0u)
-#endif
-
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t idx8_rsrc(const MapDev& m) {
   // the compact map is built only below 4 GiB; 0x00020000: 32-bit data format
   const unsigned bytes = ((unsigned)((m.width + 7) >> 3) * (unsigned)m.nby) << 6;
@@ -208,13 +199,9 @@ __device__ __forceinline__ unsigned idx8_code(const MapDev& m, const Box& box, u
   }
   // 32-bit offset from the map's base (the compact map is built only below
   // 4 GiB, so 64 * nby < 2^24): a multiply-add on the march's chain and a
-  // scalar-base load
+  // load through the map's buffer resource (scalar base, range checked)
   const unsigned off = __umul24(ux >> 3, (unsigned)m.nby << 6) + ((uy << 3) | (ux & 7u));
-#if RL_IDX8_BUFFER
   return __builtin_amdgcn_raw_buffer_load_b8(idx8_rsrc(m), off, 0, 0);
-#else
-  return glb(m.idx8)[off];
-#endif
 }
 
 // March step max(d * 0.999f, 1.0f) from a cell with non-zero code c: from the
@@ -235,11 +222,13 @@ __device__ __forceinline__ float idx8_step(const MapDev& m, const float* lut, un
 // plain truncation.  Identical results to rm_march: a cell's code is 0 exactly
 // when its distance is 0, and any other code maps back to the exact distance.
 //
-// Each loop has one exit per round: the ways a ray ends (past max_range, off
-// the map, on an occupied cell) are decided with selects, and only the kind of
-// end (1 = max_range, 2 = hit) and its cell leave the loop; the exec-mask
-// bookkeeping of three divergent exits per step cost a quarter of the kernel.
-// Off-map positions load cell (0, 0), which every map has, and ignore it.
+// Each loop has one way out: the ways a ray ends (past max_range, off the map,
+// on an occupied cell) are decided with selects, and only the kind of end
+// (1 = max_range, 2 = hit, 3 = handed to the whole wave) leaves the loop, with
+// t at the ending position (a hit's cell is formed again from it); the
+// exec-mask bookkeeping and register copies of several exits per round cost a
+// quarter of the kernel.  Off-map positions are read through the map's buffer
+// resource (0 or an ignored cell) and never used.
 //
 // Speculation (RL_SPEC > 1), after a ray's first RL_SPEC_AFTER reads: each
 // round loads the cells at t, t + g, (t + g) + g, ... (RL_SPEC positions, all
@@ -274,7 +263,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
     const bool live = t < max_range;
     const bool inb = (unsigned)px < W && (unsigned)py < H;
-    const unsigned c = idx8_code<BOX>(m, box, RL_CELL(inb, px), RL_CELL(inb, py));
+    const unsigned c = idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py);
     const int e = (!live || !inb) ? 1 : (c == 0u ? 2 : 0);
     if (COUNT) nr += (live && inb) ? 1 : 0;
     if (e) {
@@ -298,8 +287,8 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
     const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
     const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
-    const unsigned c0 = idx8_code<BOX>(m, box, RL_CELL(in0, px0), RL_CELL(in0, py0));
-    const unsigned c1 = idx8_code<BOX>(m, box, RL_CELL(in1, px1), RL_CELL(in1, py1));
+    const unsigned c0 = idx8_code<BOX>(m, box, (unsigned)px0, (unsigned)py0);
+    const unsigned c1 = idx8_code<BOX>(m, box, (unsigned)px1, (unsigned)py1);
     const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
     const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
     const bool use1 = e0 == 0 && c0 == gcode;  // position 1 is the true next position
@@ -326,9 +315,9 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const bool in0 = (unsigned)px0 < W && (unsigned)py0 < H;
     const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
     const bool in2 = (unsigned)px2 < W && (unsigned)py2 < H;
-    const unsigned c0 = idx8_code<BOX>(m, box, RL_CELL(in0, px0), RL_CELL(in0, py0));
-    const unsigned c1 = idx8_code<BOX>(m, box, RL_CELL(in1, px1), RL_CELL(in1, py1));
-    const unsigned c2 = idx8_code<BOX>(m, box, RL_CELL(in2, px2), RL_CELL(in2, py2));
+    const unsigned c0 = idx8_code<BOX>(m, box, (unsigned)px0, (unsigned)py0);
+    const unsigned c1 = idx8_code<BOX>(m, box, (unsigned)px1, (unsigned)py1);
+    const unsigned c2 = idx8_code<BOX>(m, box, (unsigned)px2, (unsigned)py2);
     const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
     const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
     const int e2 = (!(t2 < max_range) || !in2) ? 1 : (c2 == 0u ? 2 : 0);
@@ -361,7 +350,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     }
 #pragma unroll
     for (int k = 0; k < RL_SPEC; ++k)
-      c[k] = idx8_code<BOX>(m, box, RL_CELL(in[k], px[k]), RL_CELL(in[k], py[k]));
+      c[k] = idx8_code<BOX>(m, box, (unsigned)px[k], (unsigned)py[k]);
 #pragma unroll
     for (int k = 0; k < RL_SPEC; ++k) e[k] = (!(tk[k] < max_range) || !in[k]) ? 1 : (c[k] == 0u ? 2 : 0);
     int sel = RL_SPEC - 1;
@@ -422,12 +411,8 @@ __device__ __forceinline__ float rm_cast_idx8_t(const MapDev& m, const float* lu
   const unsigned W = (unsigned)m.width, H = (unsigned)m.height;
   const int px = (int)x0, py = (int)y0;
   const bool in = (unsigned)px < W && (unsigned)py < H;
-#if RL_IDX8_BUFFER
   const unsigned c_any = idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py);
   const unsigned c = in ? c_any : 0u;
-#else
-  const unsigned c = in ? idx8_code<BOX>(m, box, (unsigned)px, (unsigned)py) : 0u;
-#endif
   float dy, dx;
   sincos_libc(heading, trig, &dy, &dx);
   if (COUNT) *reads = 0;
@@ -482,7 +467,7 @@ __device__ __forceinline__ float solo_tail(const MapDev& m, const float* lut, fl
     const bool inb = (unsigned)px < W && (unsigned)py < H;
     // positions past max_range or off the map end the ray (decided before the load)
     const unsigned long long out = __builtin_amdgcn_ballot_w64(!(tk < max_range) || !inb) & lanes;
-    const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, RL_CELL(inb, px), RL_CELL(inb, py));
+    const unsigned c = idx8_code<false>(m, Box{nullptr, 0, 0}, (unsigned)px, (unsigned)py);
     // a cell that is not g's code (a hit's code 0 never is) or an ending position stops the run
     const unsigned long long stop = (__builtin_amdgcn_ballot_w64(c != gcode) & lanes) | out;
     const int j = stop ? __builtin_ctzll(stop) : RL_SOLO_W - 1;
