@@ -289,16 +289,19 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const bool in1 = (unsigned)px1 < W && (unsigned)py1 < H;
     const unsigned c0 = idx8_code<BOX>(m, box, (unsigned)px0, (unsigned)py0);
     const unsigned c1 = idx8_code<BOX>(m, box, (unsigned)px1, (unsigned)py1);
-    const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
-    const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
-    const bool use1 = e0 == 0 && c0 == gcode;  // position 1 is the true next position
-    const int es = use1 ? e1 : e0;
+    // position k is readable when it is live and on the map; a readable cell
+    // with g's code continues the run (g's code is never a hit's 0), so
+    // position 1 is the true next position exactly then
+    const bool ok0 = (t0 < max_range) & in0, ok1 = (t1 < max_range) & in1;
+    const bool use1 = ok0 & (c0 == gcode);
+    const bool oks = use1 ? ok1 : ok0;
+    const unsigned cs = use1 ? c1 : c0;
+    const float ts = use1 ? t1 : t0;
+    const int es = !oks ? 1 : (cs == 0u ? 2 : 0);
     // positions before the resolved one were read and continued; the resolved
     // one was read unless it is past max_range or off the map (es == 1)
     if (COUNT) nr += (use1 ? 1 : 0) + (es != 1 ? 1 : 0);
     const int sx = use1 ? px1 : px0, sy = use1 ? py1 : py0;
-    const unsigned cs = use1 ? c1 : c0;
-    const float ts = use1 ? t1 : t0;
     if (es) {
       end = es, t = ts;
       break;
