@@ -311,6 +311,10 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     t = ts + g;
   }
   while (RL_SPEC == 3 && !end) {  // width 3, written out
+    if (SOLO && __popcll(__builtin_amdgcn_read_exec()) == 1) {
+      end = 3;  // the wave's last marching lane: handed to the whole wave after the loops
+      break;
+    }
     const float t0 = t, t1 = t + g, t2 = t1 + g;
     const int px0 = (int)(x0 + dx * t0), py0 = (int)(y0 + dy * t0);
     const int px1 = (int)(x0 + dx * t1), py1 = (int)(y0 + dy * t1);
@@ -321,16 +325,15 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     const unsigned c0 = idx8_code<BOX>(m, box, (unsigned)px0, (unsigned)py0);
     const unsigned c1 = idx8_code<BOX>(m, box, (unsigned)px1, (unsigned)py1);
     const unsigned c2 = idx8_code<BOX>(m, box, (unsigned)px2, (unsigned)py2);
-    const int e0 = (!(t0 < max_range) || !in0) ? 1 : (c0 == 0u ? 2 : 0);
-    const int e1 = (!(t1 < max_range) || !in1) ? 1 : (c1 == 0u ? 2 : 0);
-    const int e2 = (!(t2 < max_range) || !in2) ? 1 : (c2 == 0u ? 2 : 0);
-    const bool u1 = e0 == 0 && c0 == gcode;
-    const bool u2 = u1 && e1 == 0 && c1 == gcode;
-    const int es = u2 ? e2 : (u1 ? e1 : e0);
-    if (COUNT) nr += (u1 ? 1 : 0) + (u2 ? 1 : 0) + (es != 1 ? 1 : 0);
-    const int sx = u2 ? px2 : (u1 ? px1 : px0), sy = u2 ? py2 : (u1 ? py1 : py0);
+    const bool ok0 = (t0 < max_range) & in0, ok1 = (t1 < max_range) & in1, ok2 = (t2 < max_range) & in2;
+    const bool u1 = ok0 & (c0 == gcode);
+    const bool u2 = u1 & ok1 & (c1 == gcode);
+    const bool oks = u2 ? ok2 : (u1 ? ok1 : ok0);
     const unsigned cs = u2 ? c2 : (u1 ? c1 : c0);
     const float ts = u2 ? t2 : (u1 ? t1 : t0);
+    const int es = !oks ? 1 : (cs == 0u ? 2 : 0);
+    if (COUNT) nr += (u1 ? 1 : 0) + (u2 ? 1 : 0) + (es != 1 ? 1 : 0);
+    const int sx = u2 ? px2 : (u1 ? px1 : px0), sy = u2 ? py2 : (u1 ? py1 : py0);
     if (es) {
       end = es, t = ts;
       break;
