@@ -778,9 +778,21 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
   float fw = (float)(unsigned)W;
   float fh = (float)(unsigned)H;
   int xend = trunc_x86(x1 + xstep);
-  // The reference loop only ends for finite end points below 2^23 (beyond,
-  // unit steps stop being exact and it never terminates); such rays give NaN.
-  if (!(fabsf(x0) < 8388608.0f && fabsf(x1) < 8388608.0f)) {
+  // The reference's loop runs while (int)_x != xend, both as x86 conversions
+  // (INT_MIN for NaN or out-of-range values).  It does not run at all when
+  // they agree at the start (a NaN origin, an origin beyond int range on the
+  // walking axis): max_range.  Otherwise it ends when the unit steps of _x are
+  // exact: for end points below 2^23, or for an integer start and end points
+  // below 2^24 (every float there with a fractional part is beyond reach, and
+  // every integer is exact).  Past that a step can round to no move and the
+  // reference never returns; such rays give NaN here.
+  if (trunc_x86(x0) == xend) {
+    if (COUNT) *visits = 0;
+    return max_range;
+  }
+  const bool exact = (fabsf(x0) < 8388608.0f && fabsf(x1) < 8388608.0f) ||
+                     (x0 == __builtin_truncf(x0) && fabsf(x0) < 16777216.0f && fabsf(x1) < 16777214.0f);
+  if (!exact) {
     if (COUNT) *visits = 0;
     return __builtin_nanf("");
   }
@@ -808,10 +820,11 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
     _y = adv ? _y + ystep : _y;
     error = adv ? error - deltax : error;
     const bool inb = (0.0f <= _y) & (_y < limy) & (0.0f <= _x) & (_x < limx);  // no short-circuit branch
-    // _x stays below 2^23 + 2 in magnitude (checked above); _y, off the map,
-    // may not: it is clamped into int range before the conversion (the
-    // clamp leaves every position on the map unchanged)
-    const float ym = __builtin_amdgcn_fmed3f(_y, -1.0f, 1073741824.0f);
+    // _x stays below 2^24 in magnitude (checked above); _y, off the map, may
+    // not (or be NaN, from a NaN origin): it is clamped into int range before
+    // the conversion (fminf/fmaxf return the number for a NaN; the clamp leaves
+    // every position on the map unchanged)
+    const float ym = fmaxf(fminf(_y, 1073741824.0f), -1.0f);
     const int cx = steep ? (int)_x : (int)ym;
     const int cy = steep ? (int)ym : (int)_x;
     const unsigned o = __builtin_amdgcn_raw_buffer_load_b8(occ_rsrc, __umul24((unsigned)cx, (unsigned)H) + (unsigned)cy, 0, 0);

@@ -330,6 +330,27 @@ def test_bresenham_all_rays(corridor_map, R):
         assert_bits_equal(out.cpu().numpy(), want, f"{name} R={R}")
 
 
+def test_bresenham_far_origins(ctx, rect_map):
+    """Bresenham from origins far off the map along either axis (up to 3e9,
+    beyond int range: the reference's loop then does not run, or walks exact
+    integer steps below 2^24), NaN origins (no walk: max_range), on the border
+    and just inside, with headings along and across the axes: every range equals
+    the oracle's, which walks like the reference (none of these inputs makes the
+    reference loop forever).  The walk converts the minor coordinate to a cell
+    only after clamping it, so no conversion leaves int range."""
+    m = rl.DeviceMap(ctx, O.xmajor(rect_map))
+    heads = np.array([0.0, 0.3, 1.2, np.pi / 2, 2.0, np.pi, -0.7, -np.pi / 2, 4.0], np.float32)
+    origins = np.array([[3e9, 50.0], [50.0, 3e9], [-3e9, 200.0], [200.0, -3e9], [1e7, 100.0], [100.0, -1e7],
+                        [5e6, 5e6], [-40.0, 100.0], [100.0, 600.0], [511.0, 0.0], [0.0, 511.0], [250.5, 250.5],
+                        [np.nan, 100.0], [100.0, np.nan]], np.float32)
+    q = np.array([[x, y, h] for x, y in origins for h in heads], np.float32)
+    n = q.shape[0]
+    for R in (1000.0, 50.0):
+        out = np.empty(n, np.float32)
+        m.calc_range_many(rl.RL_METHOD_BL, R, q, out, n)
+        assert_bits_equal(out, O.Oracle(rect_map, R, "bl").calc_range_many(q), f"R={R}")
+
+
 # --------------------------------------------------------------------------- step counts
 
 def test_step_counts_match_oracle(corridor_map):
