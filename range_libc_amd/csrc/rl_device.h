@@ -164,11 +164,13 @@ __device__ __forceinline__ Box stage_box(const MapDev& m, uint8_t* lds, float x0
 }
 
 // Speculation width and start.  Measured on a 2048^2 corridor map, 4000
-// particles x 60 beams, with the single-exit loops: plain 29.7 us per step;
-// width 2 after 8 or 12 reads 25.1, 25.2 us; width 3 after 4, 8 or 12 reads
-// 25.7, 25.0, 24.9 us.  Earlier speculation
-// doubles the loads of the bulk of the launch; later speculation leaves the
-// long rays' chains (runs of equal steps along corridor walls) unshortened.
+// particles x 60 beams, with the first single-exit loops: plain 29.7 us per
+// step; width 2 after 8 or 12 reads 25.1, 25.2 us; width 3 after 4, 8 or 12
+// reads 25.7, 25.0, 24.9 us.  With the whole-wave finish and the current loop
+// forms: width 2 after 6, 8 or 12 reads 20.45, 20.2, 20.38 us; width 3 after 6
+// or 8 reads 21.2, 20.8 us.  Earlier speculation doubles the loads of the bulk
+// of the launch; later speculation leaves the long rays' chains (runs of equal
+// steps along corridor walls) unshortened.
 #ifndef RL_SPEC
 #define RL_SPEC 2  // positions loaded per round of the speculative march (1: plain)
 #endif
