@@ -747,15 +747,36 @@ __device__ __forceinline__ bool occupied(const uint8_t* __restrict__ occ, int W,
   return glb(occ)[__umul24((unsigned)x, (unsigned)H) + (unsigned)y] != 0;  // W, H <= 65535: 32-bit
 }
 
-// BresenhamsLine::calc_range (RangeLib.h:696-769).  (s, c) = (sinf h, cosf h).
-template <bool COUNT>
-__device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W, int H,
-                                          float max_range, float x, float y, float s, float c,
-                                          int* visits) {
-  int nv = 0;
+#ifndef RL_BL_UNROLL
+#define RL_BL_UNROLL 8  // Bresenham steps (cell reads) per round of the walk
+#endif
+
+// A Bresenham walk in progress: BresenhamsLine::calc_range's loop state
+// (RangeLib.h:727-768) with the walk's constants.
+struct BlWalk {
+  float x, y, err;           // _x, _y, error
+  float deltax, deltay, xstep, ystep;
+  float limx, limy;          // bounds of _x and _y on the map
+  float x0, y0;              // the walk's start (axes as walked)
+  unsigned sx, sy;           // byte offset of the cell at (_x, _y): (int)_x * sx + (int)_y * sy
+  int xend, left;            // the loop's end value of (int)_x; steps still allowed
+};
+
+// Byte reads of the occupancy grid through a buffer resource covering it: a
+// read past its end returns 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t occ_rsrc(const uint8_t* occ, int W, int H) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)occ, (short)0, (int)((unsigned)W * (unsigned)H), 0x00020000);
+}
+
+// BresenhamsLine::calc_range (RangeLib.h:696-769) up to its loop, from
+// (x, y) with (s, c) = (sinf h, cosf h): true with *w set when the loop runs,
+// else false with the range in *r (an occupied start, a loop that does not
+// run, or a walk the reference never ends).
+__device__ __forceinline__ bool bl_begin(const uint8_t* __restrict__ occ, int W, int H, float max_range, float x,
+                                         float y, float s, float c, BlWalk* w, float* r) {
   if (occupied(occ, W, H, trunc_x86(x), trunc_x86(y))) {
-    if (COUNT) *visits = 0;
-    return 0.0f;
+    *r = 0.0f;
+    return false;
   }
   float x0 = y;
   float y0 = x;
@@ -770,16 +791,8 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
     x1 = y1;
     y1 = tmp;
   }
-  float deltax = fabsf(x1 - x0);
-  float deltay = fabsf(y1 - y0);
-  float error = 0.0f;
-  float _x = x0;
-  float _y = y0;
-  float xstep = (x0 < x1) ? 1.0f : -1.0f;
-  float ystep = (y0 < y1) ? 1.0f : -1.0f;
-  float fw = (float)(unsigned)W;
-  float fh = (float)(unsigned)H;
-  int xend = trunc_x86(x1 + xstep);
+  const float xstep = (x0 < x1) ? 1.0f : -1.0f;
+  const int xend = trunc_x86(x1 + xstep);
   // The reference's loop runs while (int)_x != xend, both as x86 conversions
   // (INT_MIN for NaN or out-of-range values).  It does not run at all when
   // they agree at the start (a NaN origin, an origin beyond int range on the
@@ -789,58 +802,124 @@ __device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W
   // every integer is exact).  Past that a step can round to no move and the
   // reference never returns; such rays give NaN here.
   if (trunc_x86(x0) == xend) {
-    if (COUNT) *visits = 0;
-    return max_range;
+    *r = max_range;
+    return false;
   }
   const bool exact = (fabsf(x0) < 8388608.0f && fabsf(x1) < 8388608.0f) ||
                      (x0 == __builtin_truncf(x0) && fabsf(x0) < 16777216.0f && fabsf(x1) < 16777214.0f);
   if (!exact) {
-    if (COUNT) *visits = 0;
-    return __builtin_nanf("");
+    *r = __builtin_nanf("");
+    return false;
   }
+  const float fw = (float)(unsigned)W;
+  const float fh = (float)(unsigned)H;
+  w->x = x0;
+  w->y = y0;
+  w->err = 0.0f;
+  w->deltax = fabsf(x1 - x0);
+  w->deltay = fabsf(y1 - y0);
+  w->xstep = xstep;
+  w->ystep = (y0 < y1) ? 1.0f : -1.0f;
+  // when steep the roles of (x, y) swap back for the map lookup
+  w->limx = steep ? fw : fh;
+  w->limy = steep ? fh : fw;
+  w->x0 = x0;
+  w->y0 = y0;
+  w->sx = steep ? (unsigned)H : 1u;
+  w->sy = steep ? 1u : (unsigned)H;
+  w->xend = xend;
   // (int)_x moves by one cell per step, and may repeat once while crossing
-  // zero, so the loop below ends within |xend - (int)x0| + 2 steps.
-  int cap = abs(xend - trunc_x86(x0)) + 2;
-  // When steep the roles of (x, y) swap back for the map lookup.
-  float limx = steep ? fw : fh;  // bound on _x
-  float limy = steep ? fh : fw;  // bound on _y
-  // One way out of the loop: the body has no branch (the cell is read through
-  // a buffer resource covering the map, so a step off the map reads 0 or an
-  // ignored cell), and the loop test is the reference's, on the updated _x.
-  const __amdgpu_buffer_rsrc_t occ_rsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void*)occ, (short)0, (int)((unsigned)W * (unsigned)H), 0x00020000);
-  bool hit = false;
-  int it = 0;
-  bool go = trunc_x86(_x) != xend && it < cap;
-  while (go) {
-    _x += xstep;
-    error += deltay;
-    if (COUNT) ++nv;
+  // zero, so the loop ends within |xend - (int)x0| + 2 steps.
+  w->left = abs(xend - trunc_x86(x0)) + 2;
+  return true;
+}
+
+// RL_BL_UNROLL steps of the walk's loop: true when one of them ends it (then
+// *hit says whether on an occupied cell, and (w.x, w.y) is that step's
+// position); *nv counts the steps the reference takes.  The cells a walk
+// visits do not depend on what it reads, so the round takes its steps, issues
+// their cell reads together, and then stops at the first step that ends the
+// reference's loop (a hit, or the loop test on the updated _x); the steps
+// after it are discarded.  No branch in the body: a step off the map, or past
+// the walk's end, reads 0 or a cell that is then ignored.
+template <bool COUNT>
+__device__ __forceinline__ bool bl_round(__amdgpu_buffer_rsrc_t rsrc, BlWalk& w, bool* hit, int* nv) {
+  float xs[RL_BL_UNROLL], ys[RL_BL_UNROLL];
+  bool inb[RL_BL_UNROLL], fin[RL_BL_UNROLL];
+  unsigned o[RL_BL_UNROLL];
+#pragma unroll
+  for (int k = 0; k < RL_BL_UNROLL; ++k) {
+    w.x += w.xstep;
+    w.err += w.deltay;
     // error * 2.00 >= deltax is evaluated in double (RangeLib.h:749); doubling
     // a float is exact, so the float comparison is the same test.
-    const bool adv = error * 2.0f >= deltax;
-    _y = adv ? _y + ystep : _y;
-    error = adv ? error - deltax : error;
-    const bool inb = (0.0f <= _y) & (_y < limy) & (0.0f <= _x) & (_x < limx);  // no short-circuit branch
-    // _x stays below 2^24 in magnitude (checked above); _y, off the map, may
-    // not (or be NaN, from a NaN origin): it is clamped into int range before
-    // the conversion (fminf/fmaxf return the number for a NaN; the clamp leaves
-    // every position on the map unchanged)
-    const float ym = fmaxf(fminf(_y, 1073741824.0f), -1.0f);
-    const int cx = steep ? (int)_x : (int)ym;
-    const int cy = steep ? (int)ym : (int)_x;
-    const unsigned o = __builtin_amdgcn_raw_buffer_load_b8(occ_rsrc, __umul24((unsigned)cx, (unsigned)H) + (unsigned)cy, 0, 0);
-    hit = inb & (o != 0u);
-    ++it;
-    go = !hit & (trunc_x86(_x) != xend) & (it < cap);
+    const bool adv = w.err * 2.0f >= w.deltax;
+    w.y = adv ? w.y + w.ystep : w.y;
+    w.err = adv ? w.err - w.deltax : w.err;
+    inb[k] = (0.0f <= w.y) & (w.y < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
+    // _x stays below 2^24 in magnitude (checked at the start; a few steps past
+    // the end only round); _y, off the map, may not (or be NaN, from a NaN
+    // origin): it is clamped into int range before the conversion
+    // (fminf/fmaxf return the number for a NaN; the clamp leaves every
+    // position on the map unchanged)
+    const float ym = fmaxf(fminf(w.y, 1073741824.0f), -1.0f);
+    o[k] = __builtin_amdgcn_raw_buffer_load_b8(rsrc, __umul24((unsigned)(int)w.x, w.sx) + __umul24((unsigned)(int)ym, w.sy),
+                                               0, 0);
+    xs[k] = w.x;
+    ys[k] = w.y;
+    // the reference's loop test after this step, but for the hit
+    fin[k] = ((int)w.x == w.xend) | (k + 1 >= w.left);
   }
-  if (COUNT) *visits = nv;
+  // the first step of the round that ends the walk, by selects from the last
+  bool stop = false, hk = false;
+  float ex = w.x, ey = w.y;
+  int n = RL_BL_UNROLL;
+#pragma unroll
+  for (int k = RL_BL_UNROLL - 1; k >= 0; --k) {
+    const bool h = inb[k] & (o[k] != 0u);
+    const bool s = h | fin[k];
+    ex = s ? xs[k] : ex;
+    ey = s ? ys[k] : ey;
+    hk = s ? h : hk;
+    n = s ? k + 1 : n;
+    stop |= s;
+  }
+  if (COUNT) *nv += n;
+  w.x = ex;
+  w.y = ey;
+  w.left -= RL_BL_UNROLL;
+  *hit = hk;
+  return stop;
+}
+
+// The range of a walk that ended (RangeLib.h:762-768).
+__device__ __forceinline__ float bl_range(const BlWalk& w, bool hit, float max_range) {
   if (hit) {
-    float xd = _x - x0;
-    float yd = _y - y0;
+    const float xd = w.x - w.x0;
+    const float yd = w.y - w.y0;
     return __builtin_sqrtf(xd * xd + yd * yd);
   }
   return max_range;
+}
+
+// BresenhamsLine::calc_range (RangeLib.h:696-769).  (s, c) = (sinf h, cosf h).
+template <bool COUNT>
+__device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W, int H,
+                                          float max_range, float x, float y, float s, float c,
+                                          int* visits) {
+  BlWalk w;
+  float r;
+  if (!bl_begin(occ, W, H, max_range, x, y, s, c, &w, &r)) {
+    if (COUNT) *visits = 0;
+    return r;
+  }
+  const __amdgpu_buffer_rsrc_t rsrc = occ_rsrc(occ, W, H);
+  int nv = 0;
+  bool hit = false;
+  for (;;)  // one way out
+    if (bl_round<COUNT>(rsrc, w, &hit, &nv)) break;
+  if (COUNT) *visits = nv;
+  return bl_range(w, hit, max_range);
 }
 
 // World-to-grid conversion of one pose (RangeLib.h:468-473).
