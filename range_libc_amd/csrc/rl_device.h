@@ -81,6 +81,16 @@ struct MapDev {
   int glt_td;
   float glt_R, glt_scale, glt_td_div_2pi;  // its max_range, max_div_limits, td / M_2PI
   CddtDev cddt;  // CDDTCast tables (null pointers when not built)
+  // Occupancy bitmap in 8x8-cell tiles, one 64-bit word each, in two
+  // orientations (null when not built): tile (tx, ty) = cells (8 tx + u,
+  // 8 ty + v), u, v < 8.  Steep Bresenham walks (walking along x) read the
+  // x-major tiles: word tx * bits_nty + ty, bit 8 u + v; the others (walking
+  // along y) read the y-major tiles from byte bits_ybase: word ty * bits_ntx +
+  // tx, bit 8 v + u.  The walking axis picks the tile row, the other axis
+  // the word in the row and the byte.
+  const uint8_t* bits;
+  unsigned bits_bytes, bits_ybase;  // size of both orientations; start of the y-major one
+  int bits_ntx, bits_nty;           // tiles along x ((W + 7) / 8) and y ((H + 7) / 8)
 };
 
 __device__ __forceinline__ void sincos_libc(float h, const TrigDev& t, float* s, float* c) {
@@ -760,6 +770,7 @@ struct BlWalk {
   float x0, y0;              // the walk's start (axes as walked)
   unsigned sx, sy;           // byte offset of the cell at (_x, _y): (int)_x * sx + (int)_y * sy
   int xend, left;            // the loop's end value of (int)_x; steps still allowed
+  unsigned bbase, brow;      // bitmap tiles of the walk's orientation: first byte, bytes per row
 };
 
 // Byte reads of the occupancy grid through a buffer resource covering it: a
@@ -772,8 +783,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t occ_rsrc(const uint8_t* occ, i
 // (x, y) with (s, c) = (sinf h, cosf h): true with *w set when the loop runs,
 // else false with the range in *r (an occupied start, a loop that does not
 // run, or a walk the reference never ends).
-__device__ __forceinline__ bool bl_begin(const uint8_t* __restrict__ occ, int W, int H, float max_range, float x,
-                                         float y, float s, float c, BlWalk* w, float* r) {
+__device__ __forceinline__ bool bl_begin(const MapDev& m, float max_range, float x, float y, float s, float c,
+                                         BlWalk* w, float* r) {
+  const uint8_t* __restrict__ occ = m.occ;
+  const int W = m.width, H = m.height;
   if (occupied(occ, W, H, trunc_x86(x), trunc_x86(y))) {
     *r = 0.0f;
     return false;
@@ -827,6 +840,9 @@ __device__ __forceinline__ bool bl_begin(const uint8_t* __restrict__ occ, int W,
   w->y0 = y0;
   w->sx = steep ? (unsigned)H : 1u;
   w->sy = steep ? 1u : (unsigned)H;
+  // steep walks step along x: x-major tiles, rows of bits_nty words
+  w->bbase = steep ? 0u : m.bits_ybase;
+  w->brow = 8u * (unsigned)(steep ? m.bits_nty : m.bits_ntx);
   w->xend = xend;
   // (int)_x moves by one cell per step, and may repeat once while crossing
   // zero, so the loop ends within |xend - (int)x0| + 2 steps.
@@ -892,6 +908,115 @@ __device__ __forceinline__ bool bl_round(__amdgpu_buffer_rsrc_t rsrc, BlWalk& w,
   return stop;
 }
 
+// The 2x2-tile window of the occupancy bitmap for the next RL_BL_UNROLL steps
+// of walk w (bl_round_bits): tile rows ta, ta + 1 along the walking axis,
+// tiles tb, tb + 1 across, starting at the tile of the walk's position and
+// extending one tile along each of its step directions (across, not before
+// tile 0).  A round moves at most 8 cells along each axis, so the window
+// covers every cell of the round on the map.  A tile row holds its two tiles
+// side by side: two 16-byte reads.
+struct BlWindow {
+  uint4 r0, r1;  // tile rows ta and ta + 1
+  int a0, ta, tb;
+};
+
+__device__ __forceinline__ BlWindow bl_window(__amdgpu_buffer_rsrc_t bits, const BlWalk& w) {
+  BlWindow v;
+  // the walk stays below 2^24 on the walking axis (bl_begin): an exact floor
+  v.a0 = (int)__builtin_floorf(w.x);
+  const int b0 = (int)fmaxf(fminf(w.y, 1073741824.0f), -1.0f);
+  v.ta = (v.a0 >> 3) - (w.xstep < 0.0f ? 1 : 0);
+  // the first tile across is never before the map's first: a 16-byte read
+  // that starts before the buffer is out of range as a whole, and would lose
+  // tile 0 with tile -1 (tile 0 then sits in the window's first half; the
+  // cells of tile -1 are off the map)
+  v.tb = max((b0 >> 3) - (w.ystep < 0.0f ? 1 : 0), 0);
+  // a window row before the map's first (each row is a read of its own, at a
+  // 32-bit offset that wraps) reads past the buffer (0) or some other tile;
+  // only off-map cells fall there
+  const unsigned off = w.bbase + (unsigned)v.ta * w.brow + ((unsigned)v.tb << 3);
+  v.r0 = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(bits, off, 0, 0));
+  v.r1 = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(bits, off + w.brow, 0, 0));
+  return v;
+}
+
+// bl_round over the occupancy bitmap: the same steps, stops and counts, with
+// the round's cells read from one window (bl_window) instead of one byte per
+// step.  The window loads are issued first and held there (a scheduling
+// barrier: left to itself the compiler sinks them to their first use), so they
+// are in flight while the round's steps are computed.  The window is regrouped
+// into 16 columns of 16 bits along the walking axis (v_perm) in the lane's
+// 32-byte slot of LDS, `win`; steps index the columns by floor(_x), which moves
+// by exactly one per step (the cell's (int)_x is the same number on the map).
+template <bool COUNT>
+__device__ __forceinline__ bool bl_round_bits(__amdgpu_buffer_rsrc_t bits, BlWalk& w, uint4* win, bool* hit,
+                                              int* nv) {
+  const BlWindow cur = bl_window(bits, w);
+  __builtin_amdgcn_sched_barrier(0);
+  float xs[RL_BL_UNROLL], ys[RL_BL_UNROLL];
+  bool inb[RL_BL_UNROLL], fin[RL_BL_UNROLL];
+  unsigned nrow[RL_BL_UNROLL], o[RL_BL_UNROLL];
+  const int nb = cur.tb << 3;
+  // the steps first: they need no reads
+#pragma unroll
+  for (int k = 0; k < RL_BL_UNROLL; ++k) {
+    w.x += w.xstep;
+    w.err += w.deltay;
+    // error * 2.00 >= deltax is evaluated in double (RangeLib.h:749); doubling
+    // a float is exact, so the float comparison is the same test.
+    const bool adv = w.err * 2.0f >= w.deltax;
+    w.y = adv ? w.y + w.ystep : w.y;
+    w.err = adv ? w.err - w.deltax : w.err;
+    inb[k] = (0.0f <= w.y) & (w.y < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
+    // the cell across, (int)_y with _y clamped into int range (NaN from a NaN
+    // origin clamps too; the clamp leaves every position on the map
+    // unchanged), as its bit in the window's column: 0..15 on the map
+    nrow[k] = (unsigned)((int)fmaxf(fminf(w.y, 1073741824.0f), -1.0f) - nb);
+    xs[k] = w.x;
+    ys[k] = w.y;
+    // the reference's loop test after this step, but for the hit
+    fin[k] = ((int)w.x == w.xend) | (k + 1 >= w.left);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // the steps before the window's first use
+  // column c (0..15) = bits of walking-axis cell 8 ta + c: the low byte from
+  // tile tb, the high byte from tile tb + 1 (bit n: cell 8 tb + n across)
+  const uint4 q0 = cur.r0, q1 = cur.r1;
+  win[0] = make_uint4(__builtin_amdgcn_perm(q0.z, q0.x, 0x05010400u), __builtin_amdgcn_perm(q0.z, q0.x, 0x07030602u),
+                      __builtin_amdgcn_perm(q0.w, q0.y, 0x05010400u), __builtin_amdgcn_perm(q0.w, q0.y, 0x07030602u));
+  win[1] = make_uint4(__builtin_amdgcn_perm(q1.z, q1.x, 0x05010400u), __builtin_amdgcn_perm(q1.z, q1.x, 0x07030602u),
+                      __builtin_amdgcn_perm(q1.w, q1.y, 0x05010400u), __builtin_amdgcn_perm(q1.w, q1.y, 0x07030602u));
+  // step k's column: c0 + (k + 1) * dc, always 1..15 (xstep > 0: c0 = a0 & 7)
+  // or 0..14 (xstep < 0: c0 = 8 + (a0 & 7)), whatever a0 is
+  const uint16_t* col = reinterpret_cast<const uint16_t*>(win) + (cur.a0 - 8 * cur.ta);
+  const int dc = w.xstep > 0.0f ? 1 : -1;
+  unsigned cv[RL_BL_UNROLL];
+#pragma unroll
+  for (int k = 0; k < RL_BL_UNROLL; ++k) cv[k] = col[(k + 1) * dc];
+  __builtin_amdgcn_sched_barrier(0);  // the column reads issue together
+#pragma unroll
+  for (int k = 0; k < RL_BL_UNROLL; ++k) o[k] = __builtin_amdgcn_ubfe(cv[k], nrow[k], 1u);
+  // the first step of the round that ends the walk, by selects from the last
+  bool stop = false, hk = false;
+  float ex = w.x, ey = w.y;
+  int n = RL_BL_UNROLL;
+#pragma unroll
+  for (int k = RL_BL_UNROLL - 1; k >= 0; --k) {
+    const bool h = inb[k] & (o[k] != 0u);
+    const bool s = h | fin[k];
+    ex = s ? xs[k] : ex;
+    ey = s ? ys[k] : ey;
+    hk = s ? h : hk;
+    n = s ? k + 1 : n;
+    stop |= s;
+  }
+  if (COUNT) *nv += n;
+  w.x = ex;
+  w.y = ey;
+  w.left -= RL_BL_UNROLL;
+  *hit = hk;
+  return stop;
+}
+
 // The range of a walk that ended (RangeLib.h:762-768).
 __device__ __forceinline__ float bl_range(const BlWalk& w, bool hit, float max_range) {
   if (hit) {
@@ -904,20 +1029,40 @@ __device__ __forceinline__ float bl_range(const BlWalk& w, bool hit, float max_r
 
 // BresenhamsLine::calc_range (RangeLib.h:696-769).  (s, c) = (sinf h, cosf h).
 template <bool COUNT>
-__device__ __forceinline__ float bl_trace(const uint8_t* __restrict__ occ, int W, int H,
-                                          float max_range, float x, float y, float s, float c,
+__device__ __forceinline__ float bl_trace(const MapDev& m, float max_range, float x, float y, float s, float c,
                                           int* visits) {
   BlWalk w;
   float r;
-  if (!bl_begin(occ, W, H, max_range, x, y, s, c, &w, &r)) {
+  if (!bl_begin(m, max_range, x, y, s, c, &w, &r)) {
     if (COUNT) *visits = 0;
     return r;
   }
-  const __amdgpu_buffer_rsrc_t rsrc = occ_rsrc(occ, W, H);
+  const __amdgpu_buffer_rsrc_t rsrc = occ_rsrc(m.occ, m.width, m.height);
   int nv = 0;
   bool hit = false;
   for (;;)  // one way out
     if (bl_round<COUNT>(rsrc, w, &hit, &nv)) break;
+  if (COUNT) *visits = nv;
+  return bl_range(w, hit, max_range);
+}
+
+// bl_trace over the occupancy bitmap (m.bits built), with the lane's 32-byte
+// window slot in LDS: the same ranges and counts.
+template <bool COUNT>
+__device__ __forceinline__ float bl_trace_bits(const MapDev& m, uint4* win, float max_range, float x, float y,
+                                               float s, float c, int* visits) {
+  BlWalk w;
+  float r;
+  if (!bl_begin(m, max_range, x, y, s, c, &w, &r)) {
+    if (COUNT) *visits = 0;
+    return r;
+  }
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)m.bits, (short)0, (int)(m.bits_bytes + 16u), 0x00020000);
+  int nv = 0;
+  bool hit = false;
+  for (;;)  // one way out
+    if (bl_round_bits<COUNT>(rsrc, w, win, &hit, &nv)) break;
   if (COUNT) *visits = nv;
   return bl_range(w, hit, max_range);
 }

@@ -278,7 +278,7 @@ __device__ __forceinline__ float cast_one(const MapDev& m, const float* lut, con
     return rl::rm_cast_idx8<COUNT>(m, lut, box, trig, max_range, x, y, heading, cnt);
   float s, c;
   rl::sincos_libc(heading, trig, &s, &c);
-  if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m.occ, m.width, m.height, max_range, x, y, s, c, cnt);
+  if (METHOD == RL_METHOD_BL) return rl::bl_trace<COUNT>(m, max_range, x, y, s, c, cnt);
   return rl::rm_march<COUNT>(m.edt, m.width, m.height, max_range, x, y, c, s, cnt);
 }
 
@@ -310,7 +310,14 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
   rl::MarchTail tail{false, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u};
   int cnt = 0;
   float d = 0.0f;
-  if (act) {
+  if (METHOD == RL_METHOD_BL && m.bits) {  // Bresenham over the occupancy bitmap, a window slot per thread
+    __shared__ uint4 win[2 * 256];
+    if (act) {
+      float s, c;
+      rl::sincos_libc(h, trig, &s, &c);
+      d = rl::bl_trace_bits<COUNT>(m, win + 2 * threadIdx.x, max_range, x, y, s, c, &cnt);
+    }
+  } else if (act) {
     d = (solo && rl::fast_march_ok(x, y, max_range))
             ? rl::rm_cast_idx8_t<false, false, true>(m, lut, rl::Box{nullptr, 0, 0}, trig, max_range, x, y, h, &cnt,
                                                      &tail)
@@ -678,6 +685,25 @@ __global__ __launch_bounds__(256) void k_glt_build(MapDev m, float max_range, in
 }
 
 // Compact index map from the fp32 distance transform (layout in rl_device.h).
+// Occupancy bitmap tiles (layout in MapDev): one thread per 8x8 block of
+// cells writes its x-major and its y-major word.  Cells past the map's edge
+// are free.
+__global__ void k_build_bits(const uint8_t* __restrict__ occ, unsigned long long* __restrict__ bits, int W, int H,
+                             int ntx, int nty) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)ntx * nty) return;
+  const int tx = (int)(i / nty), ty = (int)(i - (long long)tx * nty);
+  unsigned long long xm = 0, ym = 0;
+  for (int u = 0; u < 8 && tx * 8 + u < W; ++u)
+    for (int v = 0; v < 8 && ty * 8 + v < H; ++v)
+      if (occ[(size_t)(tx * 8 + u) * H + (size_t)(ty * 8 + v)]) {
+        xm |= 1ull << (8 * u + v);
+        ym |= 1ull << (8 * v + u);
+      }
+  bits[(size_t)tx * nty + ty] = xm;
+  bits[(size_t)ntx * nty + (size_t)ty * ntx + tx] = ym;
+}
+
 __global__ void k_build_idx8(const float* __restrict__ edt, const float* __restrict__ lut,
                              uint8_t* __restrict__ idx8, int W, int H, int nby, size_t nbytes) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -930,6 +956,8 @@ struct rl_map {
   float* d_edt = nullptr;
   uint8_t* d_occ = nullptr;
   uint8_t* d_idx8 = nullptr;  // compact blocked index map (null when disabled)
+  uint8_t* d_bits = nullptr;  // occupancy bitmap tiles, both orientations (see MapDev)
+  unsigned bits_bytes = 0;
   int* d_sched = nullptr;     // region buckets: count | offset | pad | perm
   int sched_cap = 0;
   double* d_table = nullptr;
@@ -952,7 +980,8 @@ struct rl_map {
     return MapDev{d_edt, d_occ, d_idx8, ctx->d_lut + RL_LUT_SIZE, width, height, nby, ctx->lds_box, world,
                   d_glt, glt_td, glt_R, glt_scale, glt_td_div_2pi,
                   rl::CddtDev{cddt_width, cddt_trans, cddt_cos, cddt_sin, cddt_bin_off, cddt_start, cddt_vals,
-                              cddt_td, cddt_td_div_2pi}};
+                              cddt_td, cddt_td_div_2pi},
+                  d_bits, bits_bytes, bits_bytes / 2u, (width + 7) / 8, (height + 7) / 8};
   }
 };
 
@@ -1150,6 +1179,31 @@ int build_compact(rl_map* m, unsigned flags) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(RL_ERR_HIP, "compact map build: %s", hipGetErrorString(e));
+  return RL_OK;
+}
+
+// Occupancy bitmap from m->d_occ (both orientations, 16 bytes of padding so
+// that a window read at the last tile stays in the buffer).  Maps whose bitmap
+// would pass 4 GiB keep walking the byte grid.
+int build_bits(rl_map* m) {
+  rl_ctx* c = m->ctx;
+  const int ntx = (m->width + 7) / 8, nty = (m->height + 7) / 8;
+  const size_t nbytes = (size_t)ntx * nty * 16;
+  if (nbytes + 16 > 0xFFFFFFFFull) return RL_OK;
+  if (hipMalloc(&m->d_bits, nbytes + 16) != hipSuccess) {
+    (void)hipGetLastError();
+    m->d_bits = nullptr;
+    return fail(RL_ERR_NOMEM, "device allocation of the %zu-byte occupancy bitmap failed", nbytes + 16);
+  }
+  m->bits_bytes = (unsigned)nbytes;
+  hipError_t e = hipMemsetAsync(m->d_bits + nbytes, 0, 16, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_build_bits, dim3(grid_for((long long)ntx * nty, 256)), dim3(256), 0, c->stream,
+                       (const uint8_t*)m->d_occ, (unsigned long long*)m->d_bits, m->width, m->height, ntx, nty);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(RL_ERR_HIP, "occupancy bitmap build: %s", hipGetErrorString(e));
   return RL_OK;
 }
 
@@ -1431,6 +1485,7 @@ int rl_map_create_ex(rl_ctx* c, const uint8_t* occ, int width, int height, unsig
     return fail(RL_ERR_HIP, "occupancy upload failed");
   }
   int rc = build_edt(m, flags);
+  if (rc == RL_OK) rc = build_bits(m);
   if (rc != RL_OK) {
     std::string msg = g_err;
     rl_map_destroy(m);
@@ -1549,6 +1604,7 @@ int rl_map_create_from_edt(rl_ctx* c, const uint8_t* occ, const float* edt, int 
     return fail(RL_ERR_HIP, "map upload failed");
   }
   int rc = build_compact(m, flags);
+  if (rc == RL_OK) rc = build_bits(m);
   if (rc == RL_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(RL_ERR_HIP, "map upload failed");
   if (rc != RL_OK) {
     std::string msg = g_err;
@@ -1568,6 +1624,7 @@ int rl_map_destroy(rl_map* m) {
   if (m->d_occ) (void)hipFree(m->d_occ);
   if (m->d_edt) (void)hipFree(m->d_edt);
   if (m->d_idx8) (void)hipFree(m->d_idx8);
+  if (m->d_bits) (void)hipFree(m->d_bits);
   if (m->d_sched) (void)hipFree(m->d_sched);
   if (m->d_table) (void)hipFree(m->d_table);
   if (m->d_glt) (void)hipFree(m->d_glt);
