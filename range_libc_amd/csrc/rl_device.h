@@ -766,7 +766,8 @@ __device__ __forceinline__ bool occupied(const uint8_t* __restrict__ occ, int W,
 struct BlWalk {
   float x, y, err;           // _x, _y, error
   float deltax, deltay, xstep, ystep;
-  float limx, limy;          // bounds of _x and _y on the map
+  float limx;                // bound of _x on the map
+  unsigned limy;             // bound of the cell across, (int)floor(_y), on the map
   float x0, y0;              // the walk's start (axes as walked)
   unsigned sx, sy;           // byte offset of the cell at (_x, _y): (int)_x * sx + (int)_y * sy
   int xend, left;            // the loop's end value of (int)_x; steps still allowed
@@ -826,16 +827,20 @@ __device__ __forceinline__ bool bl_begin(const MapDev& m, float max_range, float
   }
   const float fw = (float)(unsigned)W;
   const float fh = (float)(unsigned)H;
+  // A NaN start across (from a NaN origin) keeps _y NaN, off the map at every
+  // step: the same walk as one that starts a cell before the map and never
+  // moves across (y0 < y1 is false, so ystep is -1), with _y never NaN.
+  const bool ynan = y0 != y0;
   w->x = x0;
-  w->y = y0;
+  w->y = ynan ? -1.0f : y0;
   w->err = 0.0f;
   w->deltax = fabsf(x1 - x0);
-  w->deltay = fabsf(y1 - y0);
+  w->deltay = ynan ? 0.0f : fabsf(y1 - y0);
   w->xstep = xstep;
   w->ystep = (y0 < y1) ? 1.0f : -1.0f;
   // when steep the roles of (x, y) swap back for the map lookup
   w->limx = steep ? fw : fh;
-  w->limy = steep ? fh : fw;
+  w->limy = steep ? (unsigned)H : (unsigned)W;
   w->x0 = x0;
   w->y0 = y0;
   w->sx = steep ? (unsigned)H : 1u;
@@ -872,14 +877,13 @@ __device__ __forceinline__ bool bl_round(__amdgpu_buffer_rsrc_t rsrc, BlWalk& w,
     const bool adv = w.err * 2.0f >= w.deltax;
     w.y = adv ? w.y + w.ystep : w.y;
     w.err = adv ? w.err - w.deltax : w.err;
-    inb[k] = (0.0f <= w.y) & (w.y < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
     // _x stays below 2^24 in magnitude (checked at the start; a few steps past
-    // the end only round); _y, off the map, may not (or be NaN, from a NaN
-    // origin): it is clamped into int range before the conversion
-    // (fminf/fmaxf return the number for a NaN; the clamp leaves every
-    // position on the map unchanged)
-    const float ym = fmaxf(fminf(w.y, 1073741824.0f), -1.0f);
-    o[k] = __builtin_amdgcn_raw_buffer_load_b8(rsrc, __umul24((unsigned)(int)w.x, w.sx) + __umul24((unsigned)(int)ym, w.sy),
+    // the end only round); _y, off the map, may not (it is never NaN): the
+    // cell across is floor(_y), clamped into int range first (the clamp
+    // leaves every position on the map unchanged; floor is (int) there)
+    const int b = (int)__builtin_floorf(__builtin_amdgcn_fmed3f(w.y, -1.0f, 1073741824.0f));
+    inb[k] = ((unsigned)b < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
+    o[k] = __builtin_amdgcn_raw_buffer_load_b8(rsrc, __umul24((unsigned)(int)w.x, w.sx) + __umul24((unsigned)b, w.sy),
                                                0, 0);
     xs[k] = w.x;
     ys[k] = w.y;
@@ -924,7 +928,7 @@ __device__ __forceinline__ BlWindow bl_window(__amdgpu_buffer_rsrc_t bits, const
   BlWindow v;
   // the walk stays below 2^24 on the walking axis (bl_begin): an exact floor
   v.a0 = (int)__builtin_floorf(w.x);
-  const int b0 = (int)fmaxf(fminf(w.y, 1073741824.0f), -1.0f);
+  const int b0 = (int)__builtin_floorf(__builtin_amdgcn_fmed3f(w.y, -1.0f, 1073741824.0f));
   v.ta = (v.a0 >> 3) - (w.xstep < 0.0f ? 1 : 0);
   // the first tile across is never before the map's first: a 16-byte read
   // that starts before the buffer is out of range as a whole, and would lose
@@ -946,15 +950,16 @@ __device__ __forceinline__ BlWindow bl_window(__amdgpu_buffer_rsrc_t bits, const
 // barrier: left to itself the compiler sinks them to their first use), so they
 // are in flight while the round's steps are computed.  The window is regrouped
 // into 16 columns of 16 bits along the walking axis (v_perm) in the lane's
-// 32-byte slot of LDS, `win`; steps index the columns by floor(_x), which moves
-// by exactly one per step (the cell's (int)_x is the same number on the map).
+// 32-byte slot of LDS, `win`; a step's column is its cell's (int)_x.
 template <bool COUNT>
-__device__ __forceinline__ bool bl_round_bits(__amdgpu_buffer_rsrc_t bits, BlWalk& w, uint4* win, bool* hit,
-                                              int* nv) {
+__device__ __forceinline__ bool bl_round_bits(__amdgpu_buffer_rsrc_t bits, __amdgpu_buffer_rsrc_t occ, BlWalk& w,
+                                              uint4* win, bool* hit, int* nv) {
   const BlWindow cur = bl_window(bits, w);
   __builtin_amdgcn_sched_barrier(0);
+  const float x_start = w.x, y_start = w.y, err_start = w.err;
   float xs[RL_BL_UNROLL], ys[RL_BL_UNROLL];
   bool inb[RL_BL_UNROLL], fin[RL_BL_UNROLL];
+  int ix[RL_BL_UNROLL];
   unsigned nrow[RL_BL_UNROLL], o[RL_BL_UNROLL];
   const int nb = cur.tb << 3;
   // the steps first: they need no reads
@@ -967,15 +972,18 @@ __device__ __forceinline__ bool bl_round_bits(__amdgpu_buffer_rsrc_t bits, BlWal
     const bool adv = w.err * 2.0f >= w.deltax;
     w.y = adv ? w.y + w.ystep : w.y;
     w.err = adv ? w.err - w.deltax : w.err;
-    inb[k] = (0.0f <= w.y) & (w.y < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
-    // the cell across, (int)_y with _y clamped into int range (NaN from a NaN
-    // origin clamps too; the clamp leaves every position on the map
-    // unchanged), as its bit in the window's column: 0..15 on the map
-    nrow[k] = (unsigned)((int)fmaxf(fminf(w.y, 1073741824.0f), -1.0f) - nb);
+    // the cell across, floor(_y) clamped into int range (the clamp leaves
+    // every position on the map unchanged; _y is never NaN), and its bit in
+    // the window's column: 0..15 on the map
+    const int b = (int)__builtin_floorf(__builtin_amdgcn_fmed3f(w.y, -1.0f, 1073741824.0f));
+    inb[k] = ((unsigned)b < w.limy) & (0.0f <= w.x) & (w.x < w.limx);  // no short-circuit branch
+    nrow[k] = (unsigned)(b - nb);
     xs[k] = w.x;
     ys[k] = w.y;
-    // the reference's loop test after this step, but for the hit
-    fin[k] = ((int)w.x == w.xend) | (k + 1 >= w.left);
+    // (int)_x: the reference's loop test after this step (but for the hit),
+    // and the step's column on the map, where it is floor(_x)
+    ix[k] = (int)w.x;
+    fin[k] = (ix[k] == w.xend) | (k + 1 >= w.left);
   }
   __builtin_amdgcn_sched_barrier(0);  // the steps before the window's first use
   // column c (0..15) = bits of walking-axis cell 8 ta + c: the low byte from
@@ -985,16 +993,25 @@ __device__ __forceinline__ bool bl_round_bits(__amdgpu_buffer_rsrc_t bits, BlWal
                       __builtin_amdgcn_perm(q0.w, q0.y, 0x05010400u), __builtin_amdgcn_perm(q0.w, q0.y, 0x07030602u));
   win[1] = make_uint4(__builtin_amdgcn_perm(q1.z, q1.x, 0x05010400u), __builtin_amdgcn_perm(q1.z, q1.x, 0x07030602u),
                       __builtin_amdgcn_perm(q1.w, q1.y, 0x05010400u), __builtin_amdgcn_perm(q1.w, q1.y, 0x07030602u));
-  // step k's column: c0 + (k + 1) * dc, always 1..15 (xstep > 0: c0 = a0 & 7)
-  // or 0..14 (xstep < 0: c0 = 8 + (a0 & 7)), whatever a0 is
-  const uint16_t* col = reinterpret_cast<const uint16_t*>(win) + (cur.a0 - 8 * cur.ta);
-  const int dc = w.xstep > 0.0f ? 1 : -1;
+  const uint16_t* col = reinterpret_cast<const uint16_t*>(win);
+  const int c8 = 8 * cur.ta;
   unsigned cv[RL_BL_UNROLL];
 #pragma unroll
-  for (int k = 0; k < RL_BL_UNROLL; ++k) cv[k] = col[(k + 1) * dc];
+  for (int k = 0; k < RL_BL_UNROLL; ++k) cv[k] = col[(ix[k] - c8) & 15];  // an off-map step: some column
   __builtin_amdgcn_sched_barrier(0);  // the column reads issue together
 #pragma unroll
   for (int k = 0; k < RL_BL_UNROLL; ++k) o[k] = __builtin_amdgcn_ubfe(cv[k], nrow[k], 1u);
+  // On the map a step moves floor(_x) by one: a step that rounds (a fractional
+  // _x crossing a power of two upward) does too, unless the rounding carries
+  // into the next integer (a fraction within half a unit in the last place of
+  // 1), which moves it by two.  Downward steps on the map never round.  So the
+  // round's columns stay in the window (15 at most, the last step's largest)
+  // but after such a carry; that round is taken again from its start with
+  // byte reads.
+  if ((unsigned)(ix[RL_BL_UNROLL - 1] - c8) > 15u) {
+    w.x = x_start, w.y = y_start, w.err = err_start;
+    return bl_round<COUNT>(occ, w, hit, nv);
+  }
   // the first step of the round that ends the walk, by selects from the last
   bool stop = false, hk = false;
   float ex = w.x, ey = w.y;
@@ -1059,10 +1076,11 @@ __device__ __forceinline__ float bl_trace_bits(const MapDev& m, uint4* win, floa
   }
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void*)m.bits, (short)0, (int)(m.bits_bytes + 16u), 0x00020000);
+  const __amdgpu_buffer_rsrc_t occ = occ_rsrc(m.occ, m.width, m.height);
   int nv = 0;
   bool hit = false;
   for (;;)  // one way out
-    if (bl_round_bits<COUNT>(rsrc, w, win, &hit, &nv)) break;
+    if (bl_round_bits<COUNT>(rsrc, occ, w, win, &hit, &nv)) break;
   if (COUNT) *visits = nv;
   return bl_range(w, hit, max_range);
 }

@@ -351,6 +351,48 @@ def test_bresenham_far_origins(ctx, rect_map):
         assert_bits_equal(out, O.Oracle(rect_map, R, "bl").calc_range_many(q), f"R={R}")
 
 
+def test_bresenham_rounding_steps(ctx):
+    """Bresenham walks whose unit steps round: fractional origins just below a
+    power of two (32 ... 512) on the walking axis, within a few units in the
+    last place of the next integer, so that a step crossing the power of two
+    rounds into that integer and (int)_x moves by two cells.  The reference
+    walks those float steps; on a 600^2 map of 4000 scattered occupied cells
+    every range and every step count equals the oracle's."""
+    torch = _torch()
+    img = np.zeros((600, 600), np.uint8)
+    rng = np.random.default_rng(3)
+    for _ in range(4000):
+        img[rng.integers(0, 600), rng.integers(0, 600)] = 1
+    starts = []
+    for p2 in (32.0, 64.0, 128.0, 256.0, 512.0):
+        for k in range(1, 6):
+            for below in (1.0, 3.0, 7.5):
+                starts.append(np.float32(p2 - below - k * p2 * 2.0 ** -24))
+    starts = np.array(starts, np.float32)
+    # the steps do round into the next integer for some of these starts
+    x = starts.copy()
+    moved = np.zeros(len(x), bool)
+    for _ in range(9):
+        nx = (x + np.float32(1.0)).astype(np.float32)
+        moved |= np.floor(nx) - np.floor(x) == 2
+        x = nx
+    assert moved.any()
+    across = rng.uniform(5.0, 590.0, 8).astype(np.float32)
+    heads = np.array([0.0, 0.3, -0.3, 1.2, 1.5707964, 0.05, -0.05], np.float32)
+    q = np.array([p for s in starts for a in across for h in heads for p in ((s, a, h), (a, s, h))], np.float32)
+    n = q.shape[0]
+    m = rl.DeviceMap(ctx, O.xmajor(img))
+    d_q = torch.from_numpy(q).cuda()
+    out = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(n, dtype=torch.int32, device="cuda")
+    m.calc_range_many_dev(rl.RL_METHOD_BL, 500.0, d_q.data_ptr(), out.data_ptr(), n)
+    m.count_steps_dev(rl.RL_METHOD_BL, rl.RL_FRAME_GRID, 500.0, d_q.data_ptr(), 0, cnt.data_ptr(), n, 1)
+    ctx.sync()
+    want, want_cnt = O.Oracle(img, 500.0, "bl").calc_range_many(q, counts=True)
+    assert_bits_equal(out.cpu().numpy(), want, "ranges")
+    assert np.array_equal(cnt.cpu().numpy(), want_cnt)
+
+
 # --------------------------------------------------------------------------- step counts
 
 def test_step_counts_match_oracle(corridor_map):
