@@ -691,7 +691,7 @@ def bench_bresenham(args):
         "bl_cells_per_ray": cells["bl"] / NQ, "rm_reads_per_ray": cells["rm"] / NQ,
         "roofline": {"bound": "hbm", "achieved": alg["bl"] / (ms["bl"] * 1e-3) / 1e9, "peak": 8000.0,
                      "unit": "GB/s", "frac": alg["bl"] / (ms["bl"] * 1e-3) / 1e9 / 8000.0,
-                     "kernel": "k_cast<BL, world>", "alg_bytes_per_launch": alg["bl"],
+                     "kernel": "k_cast_bl<world> (occupancy bitmap, walks regrouped)", "alg_bytes_per_launch": alg["bl"],
                      "visited_cells_per_s": cells["bl"] / (ms["bl"] * 1e-3)},
         "config": {"workload": "bresenham_vs_rm_corridor2048_1m_r1000"}}
     # the instruction-issue bounds, from the committed counter pass over this launch

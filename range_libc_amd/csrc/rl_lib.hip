@@ -331,6 +331,107 @@ __global__ __launch_bounds__(256) void k_cast(MapDev m, TrigDev trig, float max_
     outs[r] = (FRAME == 0) ? d : d * scale;
 }
 
+// Regrouping, measured on 1 M Bresenham queries of a 2048^2 corridor map
+// (R = 1000): 256 threads with 1, 2, 4, 6 or 8 rounds between regroupings
+// 149, 134, 127, 129, 129 us; 128 or 512 threads (4 and 2 rounds) 133, 140 us;
+// one walk per lane to its end 133 us.  Each regrouping costs three barriers
+// and the parked state's LDS traffic, and waits for the workgroup's slowest wave.
+#ifndef RL_BL_WG
+#define RL_BL_WG 256  // threads of a k_cast_bl workgroup, whose walks are regrouped together
+#endif
+#ifndef RL_BL_PHASE
+#define RL_BL_PHASE 4  // rounds of every walk between regroupings (k_cast_bl)
+#endif
+#define RL_BL_PARK 18  // words of a parked walk: rl::BlWalk and its ray
+
+// A walk and its ray parked in slot i of LDS (one array per word, so that
+// consecutive slots are in consecutive banks).
+__device__ __forceinline__ void park_walk(unsigned (*park)[RL_BL_WG], int i, const rl::BlWalk& w, int ray) {
+  park[0][i] = __float_as_uint(w.x), park[1][i] = __float_as_uint(w.y), park[2][i] = __float_as_uint(w.err);
+  park[3][i] = __float_as_uint(w.deltax), park[4][i] = __float_as_uint(w.deltay);
+  park[5][i] = __float_as_uint(w.xstep), park[6][i] = __float_as_uint(w.ystep);
+  park[7][i] = __float_as_uint(w.limx), park[8][i] = w.limy;
+  park[9][i] = __float_as_uint(w.x0), park[10][i] = __float_as_uint(w.y0);
+  park[11][i] = w.sx, park[12][i] = w.sy, park[13][i] = (unsigned)w.xend, park[14][i] = (unsigned)w.left;
+  park[15][i] = w.bbase, park[16][i] = w.brow, park[17][i] = (unsigned)ray;
+}
+
+__device__ __forceinline__ void unpark_walk(const unsigned (*park)[RL_BL_WG], int i, rl::BlWalk* w, int* ray) {
+  w->x = __uint_as_float(park[0][i]), w->y = __uint_as_float(park[1][i]), w->err = __uint_as_float(park[2][i]);
+  w->deltax = __uint_as_float(park[3][i]), w->deltay = __uint_as_float(park[4][i]);
+  w->xstep = __uint_as_float(park[5][i]), w->ystep = __uint_as_float(park[6][i]);
+  w->limx = __uint_as_float(park[7][i]), w->limy = park[8][i];
+  w->x0 = __uint_as_float(park[9][i]), w->y0 = __uint_as_float(park[10][i]);
+  w->sx = park[11][i], w->sy = park[12][i], w->xend = (int)park[13][i], w->left = (int)park[14][i];
+  w->bbase = park[15][i], w->brow = park[16][i], *ray = (int)park[17][i];
+}
+
+// Bresenham rays (BresenhamsLine::calc_range, RangeLib.h:696-769, one per
+// thread as numpy_calc_range / calc_range_many cast them) over the occupancy
+// bitmap, with the walks of a workgroup regrouped every RL_BL_PHASE rounds:
+// the walks still running move (through LDS) to the workgroup's first
+// threads, so they fill whole waves and the other waves stop issuing.  Walk
+// lengths vary widely (on corridor maps the longest is ~20x the mean), and
+// with one walk per lane to the end most lanes sat idle while their wave's
+// longest walk ran.  A walk's state moves unchanged, so each ray's result is
+// that of one lane's walk.
+template <int FRAME>
+__global__ __launch_bounds__(RL_BL_WG) void k_cast_bl(MapDev m, TrigDev trig, float max_range,
+                                                      const float* __restrict__ ins,
+                                                      const float* __restrict__ angles,
+                                                      float* __restrict__ outs, int n_rays, int num_angles) {
+  __shared__ unsigned park[RL_BL_PARK][RL_BL_WG];
+  __shared__ uint4 win[2 * RL_BL_WG];
+  __shared__ int n_live;
+  const int tid = threadIdx.x;
+  const int r = blockIdx.x * RL_BL_WG + tid;
+  const __amdgpu_buffer_rsrc_t bits =
+      __builtin_amdgcn_make_buffer_rsrc((void*)m.bits, (short)0, (int)(m.bits_bytes + 16u), 0x00020000);
+  const __amdgpu_buffer_rsrc_t occ = rl::occ_rsrc(m.occ, m.width, m.height);
+  const float scale = (FRAME == 0) ? 1.0f : m.world.scale;
+  rl::BlWalk w{};
+  int ray = r;
+  bool live = false;
+  if (r < n_rays) {
+    float x, y, h, s, c, d;
+    ray_pose<FRAME>(m, ins, angles, r, num_angles, &x, &y, &h);
+    rl::sincos_libc(h, trig, &s, &c);
+    live = rl::bl_begin(m, max_range, x, y, s, c, &w, &d);
+    if (!live) outs[r] = (FRAME == 0) ? d : d * scale;  // ended before its loop
+  }
+  for (;;) {  // ends when no walk of the workgroup is left (the same in every thread)
+#pragma unroll 1
+    for (int j = 0; j < RL_BL_PHASE; ++j) {
+      if (live) {
+        bool hit;
+        int unused = 0;
+        if (rl::bl_round_bits<false>(bits, occ, w, win + 2 * tid, &hit, &unused)) {
+          const float d = rl::bl_range(w, hit, max_range);
+          outs[ray] = (FRAME == 0) ? d : d * scale;
+          live = false;
+        }
+      }
+    }
+    // regroup: each wave's running walks take consecutive slots
+    if (tid == 0) n_live = 0;
+    __syncthreads();
+    const unsigned long long lm = __builtin_amdgcn_ballot_w64(live);
+    int base = 0;
+    if ((tid & 63) == 0 && lm) base = atomicAdd(&n_live, (int)__popcll(lm));
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (live)
+      park_walk(park, base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(lm >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)lm, 0u)),
+                w, ray);
+    __syncthreads();
+    const int n = n_live;
+    if (n == 0) break;
+    live = tid < n;
+    if (live) unpark_walk(park, tid, &w, &ray);
+    __syncthreads();  // every slot read before the next regrouping writes
+  }
+}
+
 // Particles are laid out in wave segments of S lanes (S a power of two,
 // <= 64); lane j of a segment handles beams j, j + S, ...  The weight is the
 // product over beams in beam order, as the reference's loop forms it
@@ -1019,6 +1120,12 @@ template <int METHOD, int FRAME, bool COUNT>
 int launch_cast_t(rl_map* m, float max_range, const float* ins, const float* angles, float* outs,
                   int* counts, int n_rays, int num_angles) {
   if (n_rays == 0) return RL_OK;
+  if (METHOD == RL_METHOD_BL && !COUNT && m->d_bits) {
+    hipLaunchKernelGGL((k_cast_bl<FRAME>), dim3(grid_for(n_rays, RL_BL_WG)), dim3(RL_BL_WG), 0, m->ctx->stream,
+                       m->dev(), m->ctx->trig, max_range, ins, angles, outs, n_rays, num_angles);
+    HIP_TRY(hipGetLastError());
+    return RL_OK;
+  }
   hipLaunchKernelGGL((k_cast<METHOD, FRAME, COUNT>), dim3(grid_for(n_rays, 256)), dim3(256), 0,
                      m->ctx->stream, m->dev(), m->ctx->trig, max_range, ins, angles, outs, counts,
                      n_rays, num_angles);
