@@ -703,8 +703,10 @@ def bench_bresenham(args):
             "valu_us": ij["valu_issue_bound_us"], "salu_us": ij["salu_issue_bound_us"],
             "valu_busy_frac_at_profiled_duration": ij["valu_busy_frac"],
             "valu_busy_frac_here": ij["valu_issue_bound_us"] / (ms["bl"] * 1e3),
-            "source": "profiles/bresenham_issue_corridor2048_1m.json: rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU "
-                      "GRBM_GUI_ACTIVE over this launch; not measured in this run"}
+            "ta_busy_frac": ij.get("ta_busy_frac"), "l2_requests": ij.get("l2_requests"),
+            "source": "profiles/bresenham_issue_corridor2048_1m.json: rocprofv3 --pmc passes (SQ_INSTS_VALU, "
+                      "SQ_INSTS_SALU, GRBM_GUI_ACTIVE, TA_BUSY_avr, TCC_REQ_sum) over this launch; not measured "
+                      "in this run"}
     from oracle import oracle as O
 
     pick = np.random.default_rng(0).choice(NQ, 4000, replace=False)
