@@ -810,11 +810,14 @@ __device__ __forceinline__ bool bl_begin(const MapDev& m, float max_range, float
   // The reference's loop runs while (int)_x != xend, both as x86 conversions
   // (INT_MIN for NaN or out-of-range values).  It does not run at all when
   // they agree at the start (a NaN origin, an origin beyond int range on the
-  // walking axis): max_range.  Otherwise it ends when the unit steps of _x are
-  // exact: for end points below 2^23, or for an integer start and end points
-  // below 2^24 (every float there with a fractional part is beyond reach, and
-  // every integer is exact).  Past that a step can round to no move and the
-  // reference never returns; such rays give NaN here.
+  // walking axis): max_range.  For end points below 2^23, or an integer start
+  // and end points below 2^24 (every float there with a fractional part is
+  // beyond reach, and every integer is exact), each step moves (int)_x by one
+  // cell, or by two when the step rounds a fractional _x into the next integer
+  // at a power of two, and repeats it once while crossing zero; the loop ends
+  // unless such a skipped cell is xend (the reference then never returns, and
+  // the walk here stops at its step bound, below).  Past that a step can
+  // round to no move and the reference never returns; such rays give NaN here.
   if (trunc_x86(x0) == xend) {
     *r = max_range;
     return false;
@@ -849,8 +852,8 @@ __device__ __forceinline__ bool bl_begin(const MapDev& m, float max_range, float
   w->bbase = steep ? 0u : m.bits_ybase;
   w->brow = 8u * (unsigned)(steep ? m.bits_nty : m.bits_ntx);
   w->xend = xend;
-  // (int)_x moves by one cell per step, and may repeat once while crossing
-  // zero, so the loop ends within |xend - (int)x0| + 2 steps.
+  // (int)_x moves by at least one cell per step but once while crossing zero,
+  // so the loop ends within |xend - (int)x0| + 2 steps, or has skipped xend.
   w->left = abs(xend - trunc_x86(x0)) + 2;
   return true;
 }
