@@ -242,9 +242,10 @@ __device__ __forceinline__ float idx8_step(const MapDev& m, const float* lut, un
 // quarter of the kernel.  Off-map positions are read through the map's buffer
 // resource (0 or an ignored cell) and never used.
 //
-// Speculation (RL_SPEC > 1), after a ray's first RL_SPEC_AFTER reads: each
-// round loads the cells at t, t + g, (t + g) + g, ... (RL_SPEC positions, all
-// loads issued before any is used), where g is the last step taken, and then
+// Speculation (SPEC > 1, RL_SPEC unless a caller picks another width), after
+// a ray's first RL_SPEC_AFTER reads: each round loads the cells at t, t + g,
+// (t + g) + g, ... (SPEC positions, all loads issued before any is used),
+// where g is the last step taken, and then
 // walks them in order: position k + 1 is the true next position exactly when
 // cell k's step equals g, i.e. when its code equals the code that gave g
 // (distinct codes have distinct steps; escaped cells never match).  The walk
@@ -260,7 +261,7 @@ struct MarchTail {
   unsigned gcode;
 };
 
-template <bool COUNT, bool BOX, bool SOLO = false>
+template <bool COUNT, bool BOX, bool SOLO = false, int SPEC = RL_SPEC>
 __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* lut, const Box& box,
                                                  float max_range, float x0, float y0, float dx,
                                                  float dy, int* reads, float t, float g,
@@ -271,7 +272,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   // the loops leave t at the position that ended the ray; a hit's cell is
   // formed again from it after the loops (the same float operations)
   int end = 0;
-  for (int i = nr; RL_SPEC == 1 || i < RL_SPEC_AFTER; ++i) {
+  for (int i = nr; SPEC == 1 || i < RL_SPEC_AFTER; ++i) {
     const int px = (int)(x0 + dx * t), py = (int)(y0 + dy * t);
     const bool live = t < max_range;
     const bool inb = (unsigned)px < W && (unsigned)py < H;
@@ -286,7 +287,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     gcode = (c != RL_IDX8_ESCAPE) ? c : 256u;
     t += g;
   }
-  while (RL_SPEC == 2 && !end) {  // width 2, written out
+  while (SPEC == 2 && !end) {  // width 2, written out
     if (SOLO && __popcll(__builtin_amdgcn_read_exec()) == 1) {
       // the wave's last marching lane: handed to the whole wave after the
       // loops (one exit block for every way out of the loop: a return from
@@ -322,7 +323,7 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
     t = ts + g;
   }
-  while (RL_SPEC == 3 && !end) {  // width 3, written out
+  while (SPEC == 3 && !end) {  // width 3, written out
     if (SOLO && __popcll(__builtin_amdgcn_read_exec()) == 1) {
       end = 3;  // the wave's last marching lane: handed to the whole wave after the loops
       break;
@@ -354,32 +355,32 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
     gcode = (cs != RL_IDX8_ESCAPE) ? cs : 256u;
     t = ts + g;
   }
-  while (RL_SPEC > 3 && !end) {  // any width (slower selects than the written-out forms)
-    float tk[RL_SPEC];
-    int px[RL_SPEC], py[RL_SPEC], e[RL_SPEC];
-    unsigned c[RL_SPEC];
-    bool in[RL_SPEC];
+  while (SPEC > 3 && !end) {  // any width (slower selects than the written-out forms)
+    float tk[SPEC];
+    int px[SPEC], py[SPEC], e[SPEC];
+    unsigned c[SPEC];
+    bool in[SPEC];
 #pragma unroll
-    for (int k = 0; k < RL_SPEC; ++k) {
+    for (int k = 0; k < SPEC; ++k) {
       tk[k] = (k == 0) ? t : tk[k - 1] + g;
       px[k] = (int)(x0 + dx * tk[k]);
       py[k] = (int)(y0 + dy * tk[k]);
       in[k] = (unsigned)px[k] < W && (unsigned)py[k] < H;
     }
 #pragma unroll
-    for (int k = 0; k < RL_SPEC; ++k)
+    for (int k = 0; k < SPEC; ++k)
       c[k] = idx8_code<BOX>(m, box, (unsigned)px[k], (unsigned)py[k]);
 #pragma unroll
-    for (int k = 0; k < RL_SPEC; ++k) e[k] = (!(tk[k] < max_range) || !in[k]) ? 1 : (c[k] == 0u ? 2 : 0);
-    int sel = RL_SPEC - 1;
+    for (int k = 0; k < SPEC; ++k) e[k] = (!(tk[k] < max_range) || !in[k]) ? 1 : (c[k] == 0u ? 2 : 0);
+    int sel = SPEC - 1;
 #pragma unroll
-    for (int k = RL_SPEC - 2; k >= 0; --k)
+    for (int k = SPEC - 2; k >= 0; --k)
       if (e[k] != 0 || c[k] != gcode) sel = k;
     int es = e[0], sx = px[0], sy = py[0];
     unsigned cs = c[0];
     float ts = tk[0];
 #pragma unroll
-    for (int k = 1; k < RL_SPEC; ++k)
+    for (int k = 1; k < SPEC; ++k)
       if (sel == k) es = e[k], sx = px[k], sy = py[k], cs = c[k], ts = tk[k];
     if (COUNT) nr += sel + (es != 1 ? 1 : 0);
     if (es) {
@@ -404,7 +405,9 @@ __device__ __forceinline__ float rm_march_idx8_t(const MapDev& m, const float* l
   return max_range;
 }
 
-template <bool COUNT>
+// SPEC: positions loaded per round once speculating (1: the plain march), as
+// in rm_march_idx8_t.
+template <bool COUNT, int SPEC = RL_SPEC>
 __device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut, const Box& box,
                                                float max_range, float x0, float y0, float dx,
                                                float dy, int* reads) {
@@ -412,8 +415,9 @@ __device__ __forceinline__ float rm_march_idx8(const MapDev& m, const float* lut
     if (COUNT) *reads = 0;
     return max_range;
   }
-  if (box.lds) return rm_march_idx8_t<COUNT, true>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
-  return rm_march_idx8_t<COUNT, false>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
+  if (box.lds)
+    return rm_march_idx8_t<COUNT, true, false, SPEC>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
+  return rm_march_idx8_t<COUNT, false, false, SPEC>(m, lut, box, max_range, x0, y0, dx, dy, reads, 0.0f, 1.0f, 1u, 0);
 }
 
 // RayMarching::calc_range (RangeLib.h:927-962) over the compact map, from the
