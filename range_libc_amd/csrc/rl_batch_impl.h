@@ -64,14 +64,16 @@ __global__ __launch_bounds__(256) void k_batch(const BatchMap* __restrict__ maps
     const int p = item.first + wib * per_wave + seg;
     const int pp = p < bm.N ? p : -1;
     const BeamIn none{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, false};
+    // without the whole-wave finish of a wave's last ray: the batch is bound by
+    // its total work (64-map batch 1.310 -> 1.285 ms)
     if (bm.sensor)
-      particle_beams<METHOD, true>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
-                                   bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                   boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
+      particle_beams<METHOD, true, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
+                                          bm.ranges, bm.weights, pp, bm.M, S, j0, base,
+                                          boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
     else
-      particle_beams<METHOD, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
-                                    bm.ranges, bm.weights, pp, bm.M, S, j0, base,
-                                    boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
+      particle_beams<METHOD, false, false>(m, lut, trig, bm.max_range, bm.ins, bm.angles, bm.obs, bm.table, bm.K,
+                                           bm.ranges, bm.weights, pp, bm.M, S, j0, base,
+                                           boxes + wib * RL_BOX_BYTES, terms + wib * 64, none);
   }
 }
 

@@ -623,8 +623,11 @@ __device__ __forceinline__ BeamIn load_beam_in(const float* __restrict__ ins, co
 // eval_sensor_model (RangeLib.h:533-555) on the ranges just written -- each
 // range is read back as the separate call would (range * inv_world_scale), so
 // both outputs are bit-identical to the two calls.  Must be reached by the
-// whole wave (the product broadcasts across lanes).
-template <int METHOD, bool SENSOR>
+// whole wave (the product broadcasts across lanes).  SOLO: a wave's last
+// marching ray is finished by the whole wave (rl::solo_tail), which shortens
+// the launch's longest chain; launches bound by their total work (the multi-map
+// batch) do without it.
+template <int METHOD, bool SENSOR, bool SOLO = true>
 __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut, const TrigDev& trig,
                                                float max_range, const float* __restrict__ ins,
                                                const float* __restrict__ angles,
@@ -650,7 +653,7 @@ __device__ __forceinline__ void particle_beams(const MapDev& m, const float* lut
     box = rl::stage_box(m, box_lds, gy, gx, valid && rl::fast_march_ok(gy, gx, max_range));
   // ray marching over the compact map: a wave's last marching ray is finished
   // by the whole wave (rl::solo_tail); the same for every lane of the wave
-  const bool solo = RL_SOLO_TAIL && METHOD == RL_METHOD_RM && m.idx8 && !box.lds;
+  const bool solo = SOLO && RL_SOLO_TAIL && METHOD == RL_METHOD_RM && m.idx8 && !box.lds;
   const bool fast = rl::fast_march_ok(gy, gx, max_range);
   double w = 1.0;
   for (int c0 = 0; c0 < M; c0 += S) {
