@@ -740,6 +740,48 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
                                  j0, base, boxes + wib * RL_BOX_BYTES, terms + wib * 64, pre);
 }
 
+// GiantLUTCast's table by tiles of 64 consecutive cells (x-major order, so
+// mostly neighbours along y) and all their bins: a wave marches one bin's
+// direction from 64 neighbouring cells, rays of nearly equal length, instead
+// of 64 directions from one cell (a wave runs as long as its longest ray).
+// The tile's entries are assembled in LDS in the table's order (cell-major,
+// 64 * td entries) and stored contiguously.  Same values as k_glt_build, entry
+// by entry.  The tile (64 * td * 2 bytes) is the launch's dynamic LDS.
+#define RL_GLT_TILE 64
+#define RL_GLT_TILE_TD 256  // the largest td built by tiles (32 KiB of LDS)
+__global__ __launch_bounds__(256) void k_glt_build_tiles(MapDev m, float max_range, int td,
+                                                         const float2* __restrict__ dirs, float limits_div_max,
+                                                         uint16_t* __restrict__ out, long long ncells) {
+  __shared__ float lut[RL_LUT_SIZE];
+  extern __shared__ uint16_t tile[];  // RL_GLT_TILE * td entries
+  stage_lut<RL_METHOD_RM>(m, lut);
+  const long long ntiles = (ncells + RL_GLT_TILE - 1) / RL_GLT_TILE;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {  // the same in every thread
+    const long long first = t * RL_GLT_TILE;
+    const int nc = (int)min((long long)RL_GLT_TILE, ncells - first);
+    for (int r = threadIdx.x; r < RL_GLT_TILE * td; r += blockDim.x) {
+      const int c = r & (RL_GLT_TILE - 1), bin = r / RL_GLT_TILE;  // a wave: one bin, 64 cells
+      if (c >= nc) continue;
+      const long long cell = first + c;
+      const int x = (int)(cell / m.height), y = (int)(cell - (long long)x * m.height);
+      const float2 d = dirs[bin];
+      const float fx = (float)x, fy = (float)y;
+      int unused;
+      // the plain march, as in k_glt_build
+      float v = (m.idx8 && rl::fast_march_ok(fx, fy, max_range))
+                    ? rl::rm_march_idx8<false, 1>(m, lut, rl::Box{nullptr, 0, 0}, max_range, fx, fy, d.x, d.y,
+                                                  &unused)
+                    : rl::rm_march<false>(m.edt, m.width, m.height, max_range, fx, fy, d.x, d.y, &unused);
+      v = (v < max_range) ? v : max_range;  // std::min(max_range, r)
+      tile[c * td + bin] = (uint16_t)(unsigned)(v * limits_div_max);
+    }
+    __syncthreads();
+    uint16_t* dst = out + first * td;
+    for (int e = threadIdx.x; e < nc * td; e += blockDim.x) dst[e] = tile[e];
+    __syncthreads();  // the tile is stored before the next one is assembled
+  }
+}
+
 // GiantLUTCast's ray directions: (cosf, sinf) of angle i * step for i < td, the
 // values every ray of bin i uses (the reference evaluates them per ray; the same
 // function of the same float gives the same bits).
@@ -1652,7 +1694,14 @@ int rl_map_build_giant_lut(rl_map* m, float max_range, int theta_disc) {
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_glt_dirs, dim3((unsigned)((theta_disc + 255) / 256)), dim3(256), 0, c->stream, c->trig,
                        theta_disc, step, dirs);
-    if (total <= 0xFFFFFFFFLL)
+    const long long ncells = (long long)m->width * m->height;
+    if (theta_disc <= RL_GLT_TILE_TD) {
+      long long tiles = (ncells + RL_GLT_TILE - 1) / RL_GLT_TILE;
+      if (tiles > (1LL << 20)) tiles = 1LL << 20;
+      hipLaunchKernelGGL(k_glt_build_tiles, dim3((unsigned)tiles), dim3(256),
+                         (unsigned)(RL_GLT_TILE * theta_disc * sizeof(uint16_t)), c->stream, m->dev(), max_range,
+                         theta_disc, (const float2*)dirs, limits_div_max, m->d_glt, ncells);
+    } else if (total <= 0xFFFFFFFFLL)
       hipLaunchKernelGGL(k_glt_build<false>, dim3((unsigned)blocks), dim3(256), 0, c->stream, m->dev(), max_range,
                          theta_disc, (const float2*)dirs, limits_div_max, m->d_glt, total);
     else

@@ -37,6 +37,21 @@ def _map(ctx, img, R, td):
     return dm
 
 
+@pytest.mark.parametrize("td", [17, 256, 257])
+def test_glt_table_ragged_tiles(ctx, td):
+    """The whole table of a 70 x 53 map (3710 cells: the table is built in
+    tiles of 64 cells, the last one ragged) with 17 and 256 bins (tiled build)
+    and 257 (one entry per thread) equals the oracle's."""
+    img = np.zeros((53, 70), np.uint8)  # rows = y
+    img[[0, -1], :] = 1
+    img[:, [0, -1]] = 1
+    rng = np.random.default_rng(5)
+    img[rng.integers(1, 52, 150), rng.integers(1, 69, 150)] = 1
+    dm = _map(ctx, img, 40.0, td)
+    table = dm.giant_lut()[0]
+    assert np.array_equal(table, O.Oracle(img, 40.0, "glt", theta_disc=td).glt)
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_glt_table_and_queries_golden(ctx, golden, name):
     R, td = CASES[name]
