@@ -683,12 +683,38 @@ def bench_bresenham(args):
 
     reps = max(5, args.steps // 10)
     ms = {k: timed(k, reps) for k in meth}
+    # the default particle-filter step (same map; its particles, beams, max_range and sensor model)
+    # through either method
+    _, ins, angles, table, _, seed_noise = build_workload()
+    m.set_sensor_table(table)
+    d_ins, d_ang = torch.from_numpy(ins).to(dev), torch.from_numpy(angles).to(dev)
+    d_obs = torch.from_numpy(W.noisy_observation(np.full(BEAMS, 50.0, np.float32), seed_noise,
+                                                 max_range=MAX_RANGE)).to(dev)
+    d_rng = torch.empty(PARTICLES * BEAMS, dtype=torch.float32, device=dev)
+    d_w = torch.empty(PARTICLES, dtype=torch.float64, device=dev)
+
+    def pf_timed(method, reps):
+        def fn():
+            m.pf_step_dev(method, MAX_RANGE, d_ins.data_ptr(), d_ang.data_ptr(), d_obs.data_ptr(),
+                          d_rng.data_ptr(), d_w.data_ptr(), PARTICLES, BEAMS)
+        fn()
+        torch.cuda._sleep(50_000_000)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    pf_ms = {k: pf_timed(meth[k], max(20, args.steps)) for k in meth}
     alg = {"bl": cells["bl"] + 16 * NQ, "rm": 4 * cells["rm"] + 16 * NQ}
     result = {
         "metric": "Bresenham rays/s on 1 GPU (1 M integer-cell queries, 2048^2 corridors, R=1000)",
         "value": NQ / (ms["bl"] * 1e-3), "unit": "rays/s", "higher_is_better": True,
         "bl_ms": ms["bl"], "rm_ms": ms["rm"], "rm_rays_per_s": NQ / (ms["rm"] * 1e-3),
         "bl_cells_per_ray": cells["bl"] / NQ, "rm_reads_per_ray": cells["rm"] / NQ,
+        "pf_step_bl_ms": pf_ms["bl"], "pf_step_rm_ms": pf_ms["rm"],
         "roofline": {"bound": "hbm", "achieved": alg["bl"] / (ms["bl"] * 1e-3) / 1e9, "peak": 8000.0,
                      "unit": "GB/s", "frac": alg["bl"] / (ms["bl"] * 1e-3) / 1e9 / 8000.0,
                      "kernel": "k_cast_bl<world> (occupancy bitmap, walks regrouped)", "alg_bytes_per_launch": alg["bl"],
@@ -1061,7 +1087,7 @@ def secondary_workloads(args, legs) -> dict:
     sub = argparse.Namespace(**vars(args))
     sub.steps, sub.warmup, sub.no_cpu = max(20, args.steps // 10), 5, True
     keys = ("metric", "value", "unit", "ms_per_step", "bl_ms", "rm_ms", "rays", "mismatches_vs_oracle",
-            "bl_sample_mismatches", "rm_sample_mismatches", "build_ms", "prune_ms_wall", "pf_step_glt_ms",
+            "bl_sample_mismatches", "rm_sample_mismatches", "pf_step_bl_ms", "build_ms", "prune_ms_wall", "pf_step_glt_ms",
             "pf_step_pcddt_ms", "pf_step_rm_ms", "surrogate_ms_per_step", "plain_ms_per_step", "train_s",
             "autotuned")
     out = {}

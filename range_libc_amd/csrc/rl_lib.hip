@@ -767,7 +767,10 @@ __global__ __launch_bounds__(256) void k_glt_build_tiles(MapDev m, float max_ran
       const float2 d = dirs[bin];
       const float fx = (float)x, fy = (float)y;
       int unused;
-      // the plain march, as in k_glt_build
+      // the plain march: the table is bound by the work of its many rays, not
+      // by the longest chain, and speculation's extra reads cost more than
+      // they save (2048^2 corridor map, 108 bins, one entry per thread: 5.31 ms
+      // plain, 6.10 ms at width 2 after 8 reads, 5.64-5.78 ms after 16 or 32)
       float v = (m.idx8 && rl::fast_march_ok(fx, fy, max_range))
                     ? rl::rm_march_idx8<false, 1>(m, lut, rl::Box{nullptr, 0, 0}, max_range, fx, fy, d.x, d.y,
                                                   &unused)
@@ -822,12 +825,12 @@ __global__ __launch_bounds__(256) void k_glt_build(MapDev m, float max_range, in
     const float2 d = dirs[i];
     const float fx = (float)x, fy = (float)y;
     int unused;
-    // the plain march: the table is bound by the work of its many rays, not by
-    // the longest chain, and speculation's extra reads cost more than they
-    // save (2048^2 corridor map, 108 bins: 5.31 ms plain, 6.10 ms at width 2
-    // after 8 reads, 5.64-5.78 ms after 16 or 32)
+    // the default speculation: the plain march instantiated here as well (as
+    // k_glt_build_tiles marches) changed the compiler's code for the particle
+    // kernels (k_particles<RM, sensor> 2073 -> 2557 VALU instructions, about
+    // 1% slower); this kernel builds only tables of more than 256 bins
     float r = (m.idx8 && rl::fast_march_ok(fx, fy, max_range))
-                  ? rl::rm_march_idx8<false, 1>(m, lut, rl::Box{nullptr, 0, 0}, max_range, fx, fy, d.x, d.y, &unused)
+                  ? rl::rm_march_idx8<false>(m, lut, rl::Box{nullptr, 0, 0}, max_range, fx, fy, d.x, d.y, &unused)
                   : rl::rm_march<false>(m.edt, m.width, m.height, max_range, fx, fy, d.x, d.y, &unused);
     r = (r < max_range) ? r : max_range;  // std::min(max_range, r)
     out[k] = (uint16_t)(unsigned)(r * limits_div_max);
