@@ -740,15 +740,20 @@ __global__ __launch_bounds__(256) void k_particles_xcd(MapDev m, TrigDev trig, f
                                  j0, base, boxes + wib * RL_BOX_BYTES, terms + wib * 64, pre);
 }
 
-// GiantLUTCast's table by tiles of 64 consecutive cells (x-major order, so
-// mostly neighbours along y) and all their bins: a wave marches one bin's
-// direction from 64 neighbouring cells, rays of nearly equal length, instead
+// GiantLUTCast's table by tiles of RL_GLT_TILE consecutive cells (x-major
+// order, so mostly neighbours along y) and all their bins: a wave marches 4
+// bins' directions from 16 neighbouring cells, rays of similar length, instead
 // of 64 directions from one cell (a wave runs as long as its longest ray).
 // The tile's entries are assembled in LDS in the table's order (cell-major,
-// 64 * td entries) and stored contiguously.  Same values as k_glt_build, entry
-// by entry.  The tile (64 * td * 2 bytes) is the launch's dynamic LDS.
-#define RL_GLT_TILE 64
-#define RL_GLT_TILE_TD 256  // the largest td built by tiles (32 KiB of LDS)
+// RL_GLT_TILE * td entries, the launch's dynamic LDS) and stored contiguously.
+// Same values as k_glt_build, entry by entry.  Measured on the 2048^2
+// corridor map with 108 bins: tiles of 4, 8, 16, 32, 64, 128, 256 cells took
+// 4.84, 3.95, 3.69, 3.81, 4.03, 5.18, 11.4 ms (one entry per thread: 5.34 ms);
+// larger tiles hold more LDS per workgroup and leave fewer of them per CU.
+#ifndef RL_GLT_TILE
+#define RL_GLT_TILE 16  // cells per tile of k_glt_build_tiles (a power of two)
+#endif
+#define RL_GLT_TILE_TD 256  // the largest td built by tiles
 __global__ __launch_bounds__(256) void k_glt_build_tiles(MapDev m, float max_range, int td,
                                                          const float2* __restrict__ dirs, float limits_div_max,
                                                          uint16_t* __restrict__ out, long long ncells) {
@@ -760,7 +765,7 @@ __global__ __launch_bounds__(256) void k_glt_build_tiles(MapDev m, float max_ran
     const long long first = t * RL_GLT_TILE;
     const int nc = (int)min((long long)RL_GLT_TILE, ncells - first);
     for (int r = threadIdx.x; r < RL_GLT_TILE * td; r += blockDim.x) {
-      const int c = r & (RL_GLT_TILE - 1), bin = r / RL_GLT_TILE;  // a wave: one bin, 64 cells
+      const int c = r & (RL_GLT_TILE - 1), bin = r / RL_GLT_TILE;  // a wave: 64 / RL_GLT_TILE bins
       if (c >= nc) continue;
       const long long cell = first + c;
       const int x = (int)(cell / m.height), y = (int)(cell - (long long)x * m.height);

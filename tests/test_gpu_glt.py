@@ -40,7 +40,7 @@ def _map(ctx, img, R, td):
 @pytest.mark.parametrize("td", [17, 256, 257])
 def test_glt_table_ragged_tiles(ctx, td):
     """The whole table of a 70 x 53 map (3710 cells: the table is built in
-    tiles of 64 cells, the last one ragged) with 17 and 256 bins (tiled build)
+    tiles of 16 cells, the last one ragged) with 17 and 256 bins (tiled build)
     and 257 (one entry per thread) equals the oracle's."""
     img = np.zeros((53, 70), np.uint8)  # rows = y
     img[[0, -1], :] = 1
